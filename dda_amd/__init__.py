@@ -1,0 +1,385 @@
+"""MI355X-native DDA matvec and Krylov solvers: Python bindings of libdda_hip.
+
+The product is the C ABI in include/dda_hip.h, implemented by hand-written
+gfx950 HIP kernels in dda_amd/csrc and built into dda_amd/_lib/libdda_hip.so.
+This module binds it with ctypes and mirrors the reference's class interfaces:
+
+  GpuMatVec        DDA/GpuMatVec.hpp:85-99   (device vectors)
+  MatVecGpu        DDA/MatVecGpu.cpp:51-68   (host vectors; apply(arg, result, conj))
+  GpuQmrCs, GpuBicgStab, GpuCgnr, GpuBicgCs  with getPolVec(einc, eps, start)
+                   DDA/IterativeSolverBase.cpp:40-103
+
+There is no CPU fallback: if the library is missing, or no GPU is present,
+every entry point raises.
+"""
+import atexit
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libdda_hip.so")
+
+OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, MAX_ITER, STAGNATION, NOT_SUPPORTED = range(8)
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "BREAKDOWN",
+                5: "MAX_ITER", 6: "STAGNATION", 7: "NOT_SUPPORTED"}
+QMR_CS, BICGSTAB, CGNR, BICG_CS = 0, 1, 2, 3
+SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_CS}
+
+# Every symbol include/dda_hip.h declares.
+EXPORTS = [
+    "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
+    "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_fft_fit",
+    "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_apply",
+    "dda_op_apply_host", "dda_op_get_dmatrix",
+    "dda_solver_create", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
+]
+
+
+class DDAError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+class SolverError(DDAError):
+    """Raised for BREAKDOWN / MAX_ITER / STAGNATION (the reference's ABORT_MSG/ASSERT)."""
+
+
+class OpDesc(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_uint32 * 3), ("grid", ctypes.c_uint32 * 3), ("nv", ctypes.c_uint32),
+                ("vec_stride", ctypes.c_uint32), ("pos", ctypes.POINTER(ctypes.c_uint32)),
+                ("mat", ctypes.POINTER(ctypes.c_uint8)), ("nmat", ctypes.c_uint32),
+                ("grid_unit", ctypes.c_double), ("wave_num", ctypes.c_double),
+                ("dmatrix", ctypes.POINTER(ctypes.c_double))]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("matvecs", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
+                ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libdda_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libdda_hip.so is missing: run `python dda_amd/build.py` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, u32, i32, u64, dbl = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_double
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.dda_last_error.restype = ctypes.c_char_p
+        L.dda_ctx_create.argtypes = [i32, pp]
+        L.dda_ctx_destroy.argtypes = [vp]
+        L.dda_ctx_synchronize.argtypes = [vp]
+        L.dda_ctx_info.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz), ctypes.POINTER(i32)]
+        L.dda_malloc.argtypes = [vp, sz, pp]
+        L.dda_free.argtypes = [vp, vp]
+        L.dda_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+        L.dda_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+        L.dda_fft_fit.argtypes = [u32, u32, i32]
+        L.dda_fft_fit.restype = u32
+        L.dda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), i32, pp]
+        L.dda_op_destroy.argtypes = [vp]
+        L.dda_op_info.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(sz)]
+        L.dda_op_set_cc_sqrt.argtypes = [vp, dp]
+        L.dda_op_apply.argtypes = [vp, vp, vp, i32, vp]
+        L.dda_op_apply_host.argtypes = [vp, dp, dp, i32]
+        L.dda_op_get_dmatrix.argtypes = [vp, dp]
+        L.dda_solver_create.argtypes = [vp, i32, u64, u64, pp]
+        L.dda_solver_destroy.argtypes = [vp]
+        L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
+        L.dda_solver_history.argtypes = [vp, dp, u64, ctypes.POINTER(u64)]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("dda_last_error", "dda_fft_fit"):
+                f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(status, what=""):
+    if status != OK:
+        msg = lib().dda_last_error().decode(errors="replace")
+        cls = SolverError if status in (BREAKDOWN, MAX_ITER, STAGNATION) else DDAError
+        raise cls(status, (what + ": " if what else "") + msg)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fft_fit(x, override=0, support_non_pot=True):
+    """DDA/DDAParams.cpp:45-90 (x = 2 * box extent)."""
+    return int(lib().dda_fft_fit(x, override, int(support_non_pot)))
+
+
+_live_contexts = weakref.WeakSet()
+
+
+class Context:
+    """One GPU and one in-order stream (DDA/DDAMain.cpp:556-575).
+
+    Objects created on a context (vectors, operators, solvers) register with it;
+    closing the context releases them first, so no handle outlives it."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().dda_ctx_create(device, ctypes.byref(h)), "dda_ctx_create")
+        self.handle = h
+        self._children = []
+        _live_contexts.add(self)
+
+    def _adopt(self, obj):
+        if len(self._children) > 256:
+            self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
+
+    def info(self):
+        name = ctypes.create_string_buffer(64)
+        hbm = ctypes.c_size_t()
+        cus = ctypes.c_int()
+        _check(lib().dda_ctx_info(self.handle, name, 64, ctypes.byref(hbm), ctypes.byref(cus)))
+        return {"arch": name.value.decode(), "hbm_bytes": hbm.value, "cu_count": cus.value}
+
+    def synchronize(self):
+        _check(lib().dda_ctx_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            # solvers before operators before plain buffers
+            live = [r() for r in reversed(self._children)]
+            for obj in live:
+                if obj is not None:
+                    obj.close()
+            self._children = []
+            lib().dda_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@atexit.register
+def _close_all_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+class DeviceVector:
+    """Plain device buffer of 3*vec_stride complex values of one precision."""
+
+    def __init__(self, ctx, n, precision):
+        self.ctx, self.n, self.precision = ctx, int(n), precision
+        self.dtype = np.complex128 if precision == 64 else np.complex64
+        self.nbytes = self.n * np.dtype(self.dtype).itemsize
+        p = ctypes.c_void_p()
+        _check(lib().dda_malloc(ctx.handle, self.nbytes, ctypes.byref(p)), "dda_malloc")
+        self.ptr = p
+        ctx._adopt(self)
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        assert h.size == self.n
+        _check(lib().dda_memcpy_h2d(self.ctx.handle, self.ptr, h.ctypes.data, self.nbytes))
+
+    def download(self):
+        h = np.empty(self.n, self.dtype)
+        _check(lib().dda_memcpy_d2h(self.ctx.handle, h.ctypes.data, self.ptr, self.nbytes))
+        return h
+
+    def close(self):
+        if self.ptr and self.ctx.handle:
+            lib().dda_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    free = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuMatVec:
+    """The FFT-convolution operator A = I - sqrt(a) G sqrt(a) on the device.
+
+    GpuMatVec::create (DDA/GpuMatVec.hpp:85-86). `pos` is [nv, 3] grid
+    coordinates, `mat` [nv] material indices, `grid` 0 entries use fftFit's
+    smooth rule. `dmatrix` optionally supplies the reference-layout D matrix
+    (complex [Gx, Gz, Gy, 6], 1/m^3); otherwise it is generated on the GPU.
+    """
+
+    def __init__(self, ctx, pos, mat, box, grid_unit, wave_num, precision=64, grid=(0, 0, 0), nmat=None,
+                 vec_stride=0, dmatrix=None):
+        self.ctx = ctx
+        self.pos = np.ascontiguousarray(pos, np.uint32).reshape(-1, 3)
+        self.nv = self.pos.shape[0]
+        self.mat = np.ascontiguousarray(mat if mat is not None else np.zeros(self.nv), np.uint8)
+        self.nmat = int(nmat if nmat is not None else (int(self.mat.max()) + 1 if self.nv else 1))
+        self.precision = precision
+        d = OpDesc()
+        d.box[:] = [int(b) for b in box]
+        d.grid[:] = [int(g) for g in grid]
+        d.nv = self.nv
+        d.vec_stride = int(vec_stride)
+        d.pos = self.pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        d.mat = self.mat.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d.nmat = self.nmat
+        d.grid_unit = float(grid_unit)
+        d.wave_num = float(wave_num)
+        self._dm = None
+        if dmatrix is not None:
+            self._dm = np.ascontiguousarray(dmatrix, np.complex128)
+            d.dmatrix = _dp(self._dm.view(np.float64))
+        h = ctypes.c_void_p()
+        _check(lib().dda_op_create(ctx.handle, ctypes.byref(d), precision, ctypes.byref(h)), "dda_op_create")
+        self.handle = h
+        ctx._adopt(self)
+        self._dm = None
+        g = (ctypes.c_uint32 * 3)()
+        vs = ctypes.c_uint32()
+        nb = ctypes.c_size_t()
+        _check(lib().dda_op_info(h, g, ctypes.byref(vs), ctypes.byref(nb)))
+        self.grid = tuple(int(v) for v in g)
+        self.vec_stride = int(vs.value)
+        self.device_bytes = int(nb.value)
+        self.box = tuple(int(b) for b in box)
+        self.grid_unit, self.wave_num = float(grid_unit), float(wave_num)
+
+    @property
+    def vec_size(self):
+        return 3 * self.vec_stride
+
+    def set_couple_constants(self, cc_sqrt):
+        """GpuMatVec::setCoupleConstants: cc_sqrt [nmat, 3] complex (m^(3/2))."""
+        cs = np.ascontiguousarray(np.asarray(cc_sqrt, np.complex128).reshape(self.nmat, 3))
+        _check(lib().dda_op_set_cc_sqrt(self.handle, _dp(cs.view(np.float64))), "dda_op_set_cc_sqrt")
+
+    def apply(self, x, y, conj=False, stream=None):
+        """Device vectors (DeviceVector or raw pointers); enqueued, not synchronised."""
+        xp = x.ptr if isinstance(x, DeviceVector) else x
+        yp = y.ptr if isinstance(y, DeviceVector) else y
+        _check(lib().dda_op_apply(self.handle, xp, yp, int(conj), stream), "dda_op_apply")
+
+    def dmatrix(self):
+        """The device tensor in the reference layout [Gx, Gz, Gy, 6] (1/m^3)."""
+        gx, gy, gz = self.grid
+        out = np.zeros((gx, gz, gy, 6), np.complex128)
+        _check(lib().dda_op_get_dmatrix(self.handle, _dp(out.view(np.float64))))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            if self.ctx.handle:
+                lib().dda_op_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatVecGpu:
+    """MatVec<T> on host vectors (DDA/MatVecGpu.cpp:51-68): apply(arg, result, conj)."""
+
+    def __init__(self, gpu_matvec):
+        self.op = gpu_matvec
+
+    def apply(self, arg, result=None, conj=False):
+        x = np.ascontiguousarray(arg, np.complex128)
+        assert x.size == self.op.vec_size
+        y = np.empty_like(x) if result is None else result
+        _check(lib().dda_op_apply_host(self.op.handle, _dp(x.view(np.float64)), _dp(y.view(np.float64)), int(conj)),
+               "dda_op_apply_host")
+        return y
+
+
+class IterativeSolver:
+    """IterativeSolverBase<T> over a GpuMatVec (FP64): getPolVec(einc, eps, start)."""
+
+    kind = QMR_CS
+
+    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0):
+        self.op = gpu_matvec
+        h = ctypes.c_void_p()
+        _check(lib().dda_solver_create(gpu_matvec.handle, self.kind, int(max_iter), int(max_res_increase),
+                                       ctypes.byref(h)), "dda_solver_create")
+        self.handle = h
+        self.info = None
+        gpu_matvec.ctx._adopt(self)
+
+    def set_couple_constants(self, cc_sqrt):
+        self.op.set_couple_constants(cc_sqrt)
+
+    def get_pol_vec(self, einc, eps, start=None):
+        e = np.ascontiguousarray(einc, np.complex128)
+        assert e.size == self.op.vec_size
+        out = np.zeros_like(e)
+        st = None if start is None else np.ascontiguousarray(start, np.complex128)
+        info = SolveInfo()
+        status = lib().dda_solver_solve(self.handle, _dp(e.view(np.float64)), float(eps),
+                                        None if st is None else _dp(st.view(np.float64)),
+                                        _dp(out.view(np.float64)), ctypes.byref(info))
+        self.info = info.as_dict()
+        _check(status, "getPolVec")
+        return out
+
+    getPolVec = get_pol_vec
+
+    def history(self):
+        n = int(self.info["iterations"]) + 1 if self.info else 1
+        out = np.zeros(n)
+        w = ctypes.c_uint64()
+        _check(lib().dda_solver_history(self.handle, _dp(out), n, ctypes.byref(w)))
+        return out[: w.value]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            if self.op.handle and self.op.ctx.handle:
+                lib().dda_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuQmrCs(IterativeSolver):
+    kind = QMR_CS
+
+
+class GpuBicgStab(IterativeSolver):
+    kind = BICGSTAB
+
+
+class GpuCgnr(IterativeSolver):
+    kind = CGNR
+
+
+class GpuBicgCs(IterativeSolver):
+    kind = BICG_CS
+
+
+SOLVER_CLASSES = {"qmr": GpuQmrCs, "bicgstab": GpuBicgStab, "cgnr": GpuCgnr, "bicg": GpuBicgCs}

@@ -1,0 +1,94 @@
+// Host-side structures shared by the translation units of libdda_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dda_hip.h"
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace dda {
+
+void set_error(const std::string& msg);
+
+#define DDA_TRY_HIP(expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      ::dda::set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));         \
+      return (e_ == hipErrorOutOfMemory) ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;             \
+    }                                                                                       \
+  } while (0)
+
+#define DDA_TRY(expr)                    \
+  do {                                   \
+    dda_status s_ = (expr);              \
+    if (s_ != DDA_OK) return s_;         \
+  } while (0)
+
+// Extents the matvec kernels need; passed by value.
+struct MvDims {
+  int Bx, By, Bz;
+  int Gx, Gy, Gz;
+  int nv;
+  int S;       // vector stride
+  int nrows;   // By * Bz
+};
+
+// Build a plan: radix-8 stages first, then 4, 2, 3, 5, 7. Returns false if n
+// has a prime factor above 7.
+bool make_plan(int n, FftPlan* p);
+
+// Device twiddle table exp(-2 pi i t / n), t < n, computed in long double.
+dda_status make_twiddles(int n, int prec, void** out);
+
+// Launch helpers implemented in matvec.hip.
+template <typename R>
+dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj, hipStream_t st,
+                         const int* done);
+
+}  // namespace dda
+
+struct dda_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop;
+};
+
+struct dda_op_s {
+  dda_ctx_s* ctx = nullptr;
+  int prec = 64;
+  dda::MvDims dims;
+  uint32_t nmat = 0;
+  double grid_unit = 0, wave_num = 0;
+  bool cc_set = false;
+  std::vector<uint32_t> pos;  // host copy [nv][3]
+  std::vector<uint8_t> mat;   // host copy [nv]
+  std::vector<double> cc_sqrt_host;  // [nmat][3] complex, reference units
+
+  // device
+  uint32_t* row_start = nullptr;  // [nrows + 1]
+  uint32_t* row_dip = nullptr;    // [nv] dipole index, grouped by row
+  uint32_t* row_meta = nullptr;   // [nv] x | mat << 16
+  uint32_t* row_list = nullptr;   // rows holding at least one dipole
+  uint8_t* mat_dev = nullptr;     // [nv] material index per dipole
+  int n_nonempty = 0;
+  void* ccs = nullptr;            // [nmat][3] complex R, d^(-3/2) scaled
+  void* dhat = nullptr;           // octant [Gz/2+1][Gy/2+1][Gx/2+1][6] complex R, d^3 scaled
+  void* a1 = nullptr;             // [3][Bz][By][Gx]
+  void* a2 = nullptr;             // [3][Bz][Gy][Gx]
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  dda::FftPlan plan[3];
+  void* stage_x = nullptr;        // host-apply staging vectors
+  void* stage_y = nullptr;
+  size_t device_bytes = 0;
+
+  // launch shapes
+  int p1_rows = 1;   // rows per work-group in the X passes
+  int col_t = 8;     // columns per work-group in the Y/Z passes
+};

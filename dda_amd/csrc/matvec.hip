@@ -1,0 +1,972 @@
+// The FFT-convolution matvec y = x - sqrt(a) G sqrt(a) x on one MI355X.
+//
+// Replaces GpuMatVec::apply (DDA/GpuMatVec.cpp:144-431) and its kernels
+// initXMatrix / innerLoop / createResVec (DDA/GpuMatVecInst.cl:31-121), the LDS
+// transposes (DDA/GpuTransposePlanInst.cl) and the Apple FFT plans
+// (DDA/GpuFFTPlanCl.cpp) with five passes over HBM and no transposes: the
+// contiguous x axis is transformed first, after which every later axis is a
+// strided column FFT over a tile of adjacent x columns, so each row of the tile
+// is one coalesced segment.
+//
+//   pass 1  X-FFT of each (z, y) row, fused with the scatter of sqrt(a) x
+//           (initXMatrix):         x -> A1[3][Bz][By][Gx]
+//   pass 2  Y-FFT, y zero-padded:   A1 -> A2[3][Bz][Gy][Gx]
+//   pass 3  Z-FFT (z zero-padded), 3x3 complex-symmetric D multiply (innerLoop),
+//           inverse Z-FFT cropped to z < Bz, in place on A2
+//   pass 4  inverse Y-FFT cropped to y < By: A2 -> A1
+//   pass 5  inverse X-FFT fused with the gather y = sqrt(a) X + x (createResVec)
+//
+// The FFT'd interaction tensor is stored as one octant: G_xx, G_yy, G_zz are
+// even in every offset component and G_xy (G_xz, G_yz) odd in x and y (x and z,
+// y and z), so D(G - k) = +-D(k) per axis (DDA/DMatrixCpu.cpp:49-63).
+
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace dda {
+
+// ------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------
+
+__device__ inline bool stop_requested(const int* done) { return done != nullptr && *done != 0; }
+
+// Pass 1: rows r0 .. r0+Lr-1 of the (z, y) plane, all three components. Tile
+// line l = c * Lr + lr.
+template <typename R, int PPT>
+__global__ __launch_bounds__(256) void k_pass1(const typename CxT<R>::T* __restrict__ x,
+                                               typename CxT<R>::T* __restrict__ a1,
+                                               const uint32_t* __restrict__ row_start,
+                                               const uint32_t* __restrict__ row_dip,
+                                               const uint32_t* __restrict__ row_meta,
+                                               const typename CxT<R>::T* __restrict__ ccs, MvDims d, int lr_n,
+                                               FftPlan px, FastDiv ldiv, FastDiv ndiv,
+                                               const typename CxT<R>::T* __restrict__ tw, int conj,
+                                               const int* done) {
+  typedef typename CxT<R>::T C;
+  if (stop_requested(done)) return;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int L = 3 * lr_n;
+  const int N = d.Gx;
+  const int pts = N * L;
+  const int r0 = blockIdx.x * lr_n;
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) tile[i] = czero<C>();
+  __syncthreads();
+  for (int lr = 0; lr < lr_n; lr++) {
+    const int row = r0 + lr;
+    if (row >= d.nrows) break;
+    const uint32_t s0 = row_start[row];
+    const int cnt = (int)(row_start[row + 1] - s0);
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+      const uint32_t i = row_dip[s0 + q];
+      const uint32_t meta = row_meta[s0 + q];
+      const int xp = meta & 0xffff;
+      const int m = meta >> 16;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        C v = x[(size_t)c * d.S + i];
+        if (conj) v = cconj(v);
+        tile[xp * L + c * lr_n + lr] = cmul(ccs[3 * m + c], v);
+      }
+    }
+  }
+  __syncthreads();
+  fft_tile<PPT, false>(tile, px, ldiv, tw);
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+    const int l = ndiv.div(i);
+    const int p = i - l * N;
+    const int c = l / lr_n;
+    const int lr = l - c * lr_n;
+    const int row = r0 + lr;
+    if (row < d.nrows) a1[((size_t)c * d.nrows + row) * N + p] = tile[p * L + l];
+  }
+}
+
+// Pass 5: rows taken from the list of non-empty rows.
+template <typename R, int PPT>
+__global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restrict__ a1,
+                                               const typename CxT<R>::T* __restrict__ x,
+                                               typename CxT<R>::T* __restrict__ y,
+                                               const uint32_t* __restrict__ row_list, int n_list,
+                                               const uint32_t* __restrict__ row_start,
+                                               const uint32_t* __restrict__ row_dip,
+                                               const uint32_t* __restrict__ row_meta,
+                                               const typename CxT<R>::T* __restrict__ ccs, MvDims d, int lr_n,
+                                               FftPlan px, FastDiv ldiv, FastDiv ndiv,
+                                               const typename CxT<R>::T* __restrict__ tw, int conj,
+                                               const int* done) {
+  typedef typename CxT<R>::T C;
+  if (stop_requested(done)) return;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int L = 3 * lr_n;
+  const int N = d.Gx;
+  const int pts = N * L;
+  const int s_base = blockIdx.x * lr_n;
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+    const int l = ndiv.div(i);
+    const int p = i - l * N;
+    const int c = l / lr_n;
+    const int lr = l - c * lr_n;
+    const int slot = s_base + lr;
+    C v = czero<C>();
+    if (slot < n_list) v = a1[((size_t)c * d.nrows + row_list[slot]) * N + p];
+    tile[p * L + l] = v;
+  }
+  __syncthreads();
+  fft_tile<PPT, true>(tile, px, ldiv, tw);
+  for (int lr = 0; lr < lr_n; lr++) {
+    const int slot = s_base + lr;
+    if (slot >= n_list) break;
+    const int row = row_list[slot];
+    const uint32_t s0 = row_start[row];
+    const int cnt = (int)(row_start[row + 1] - s0);
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+      const uint32_t i = row_dip[s0 + q];
+      const uint32_t meta = row_meta[s0 + q];
+      const int xp = meta & 0xffff;
+      const int m = meta >> 16;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        C a = x[(size_t)c * d.S + i];
+        if (conj) a = cconj(a);
+        C r = cadd(cmul(ccs[3 * m + c], tile[xp * L + c * lr_n + lr]), a);
+        if (conj) r = cconj(r);
+        y[(size_t)c * d.S + i] = r;
+      }
+    }
+  }
+  if (blockIdx.x == 0) {
+    const int pad = d.S - d.nv;
+    for (int t = threadIdx.x; t < 3 * pad; t += blockDim.x) {
+      const int c = t / pad;
+      y[(size_t)c * d.S + d.nv + (t - c * pad)] = czero<C>();
+    }
+  }
+}
+
+// Strided column FFT over T adjacent columns. Column `col` of batch b, point p
+// lives at base[b * bstride + p * pstride + col]; inputs beyond in_len are zero,
+// only the first out_len points are stored.
+struct ColParams {
+  int ncols;
+  int in_len, out_len;
+  long long pstride_in, pstride_out;
+  long long bstride_in, bstride_out;
+};
+
+template <typename R, int PPT, bool INV>
+__global__ __launch_bounds__(256) void k_colfft(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
+                                                FftPlan p, FastDiv tdiv, const typename CxT<R>::T* __restrict__ tw,
+                                                const int* done) {
+  typedef typename CxT<R>::T C;
+  if (stop_requested(done)) return;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int T = (int)tdiv.d;
+  const int c0 = blockIdx.x * T;
+  const long long b = blockIdx.y;
+  const C* src = in + b * cp.bstride_in;
+  C* dst = out + b * cp.bstride_out;
+  const int pts = p.n * T;
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+    const int pp = tdiv.div(i);
+    const int col = c0 + i - pp * T;
+    C v = czero<C>();
+    if (pp < cp.in_len && col < cp.ncols) v = src[pp * cp.pstride_in + col];
+    tile[i] = v;
+  }
+  __syncthreads();
+  fft_tile<PPT, INV>(tile, p, tdiv, tw);
+  const int opts = cp.out_len * T;
+  for (int i = threadIdx.x; i < opts; i += blockDim.x) {
+    const int pp = tdiv.div(i);
+    const int col = c0 + i - pp * T;
+    if (col < cp.ncols) dst[pp * cp.pstride_out + col] = tile[i];
+  }
+}
+
+// Pass 3 over the (ky, kx tile) pencil: for each component load z < Bz, FFT,
+// keep the spectrum in registers; multiply by D(kx, ky, kz); inverse FFT each
+// component and store z < Bz back in place.
+template <typename R, int PPT>
+__global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
+                                               MvDims d, FftPlan pz, FastDiv tdiv,
+                                               const typename CxT<R>::T* __restrict__ tw, const int* done) {
+  typedef typename CxT<R>::T C;
+  if (stop_requested(done)) return;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int T = (int)tdiv.d;
+  const int c0 = blockIdx.x * T;
+  const int ky = blockIdx.y;
+  const int N = d.Gz;
+  const int pts = N * T;
+  const long long cstride = (long long)d.Bz * d.Gy * d.Gx;
+  const long long zstride = (long long)d.Gy * d.Gx;
+  C* base = a2 + (long long)ky * d.Gx;
+  C reg[3][PPT];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+      const int z = tdiv.div(i);
+      const int col = c0 + i - z * T;
+      C v = czero<C>();
+      if (z < d.Bz && col < d.Gx) v = base[c * cstride + z * zstride + col];
+      tile[i] = v;
+    }
+    __syncthreads();
+    fft_tile<PPT, false>(tile, pz, tdiv, tw);
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int i = threadIdx.x + k * blockDim.x;
+      if (i < pts) reg[c][k] = tile[i];
+    }
+    __syncthreads();
+  }
+  const int hx = d.Gx / 2, hy = d.Gy / 2, hz = d.Gz / 2;
+  const int fy = ky <= hy ? ky : d.Gy - ky;
+  const R sy = ky <= hy ? (R)1 : (R)-1;
+#pragma unroll
+  for (int k = 0; k < PPT; k++) {
+    const int i = threadIdx.x + k * blockDim.x;
+    if (i < pts) {
+      const int kz = tdiv.div(i);
+      const int kx = c0 + i - kz * T;
+      if (kx < d.Gx) {
+        const int fx = kx <= hx ? kx : d.Gx - kx;
+        const int fz = kz <= hz ? kz : d.Gz - kz;
+        const R sx = kx <= hx ? (R)1 : (R)-1;
+        const R sz = kz <= hz ? (R)1 : (R)-1;
+        const C* D = dhat + (((long long)fz * (hy + 1) + fy) * (hx + 1) + fx) * 6;
+        const C d0 = D[0], d3 = D[3], d5 = D[5];
+        const C d1 = cscale(D[1], sx * sy), d2 = cscale(D[2], sx * sz), d4 = cscale(D[4], sy * sz);
+        const C x0 = reg[0][k], x1 = reg[1][k], x2 = reg[2][k];
+        reg[0][k] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
+        reg[1][k] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
+        reg[2][k] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int i = threadIdx.x + k * blockDim.x;
+      if (i < pts) tile[i] = reg[c][k];
+    }
+    __syncthreads();
+    fft_tile<PPT, true>(tile, pz, tdiv, tw);
+    const int opts = d.Bz * T;
+    for (int i = threadIdx.x; i < opts; i += blockDim.x) {
+      const int z = tdiv.div(i);
+      const int col = c0 + i - z * T;
+      if (col < d.Gx) base[c * cstride + z * zstride + col] = tile[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Row FFT over contiguous lines (used for the X axis of the tensor generation).
+template <typename R, int PPT, bool INV>
+__global__ __launch_bounds__(256) void k_rowfft(typename CxT<R>::T* data, long long nlines, FftPlan p, FastDiv ldiv,
+                                                FastDiv ndiv, const typename CxT<R>::T* __restrict__ tw) {
+  typedef typename CxT<R>::T C;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int L = (int)ldiv.d;
+  const int N = p.n;
+  const int pts = N * L;
+  const long long l0 = (long long)blockIdx.x * L;
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+    const int l = ndiv.div(i);
+    const int pp = i - l * N;
+    C v = czero<C>();
+    if (l0 + l < nlines) v = data[(l0 + l) * N + pp];
+    tile[pp * L + l] = v;
+  }
+  __syncthreads();
+  fft_tile<PPT, INV>(tile, p, ldiv, tw);
+  for (int i = threadIdx.x; i < pts; i += blockDim.x) {
+    const int l = ndiv.div(i);
+    const int pp = i - l * N;
+    if (l0 + l < nlines) data[(l0 + l) * N + pp] = tile[pp * L + l];
+  }
+}
+
+// d^3-scaled Green tensor component on the full grid, clipped to |offset| < clip
+// per axis (DDA/DMatrixCpu.cpp:49-63,124-131,169-193). Offsets and kd are in
+// units of the grid spacing.
+__global__ void k_green_fill(double2* w, int comp, int Gx, int Gy, int Gz, int cx, int cy, int cz, double kd) {
+  const long long total = (long long)Gx * Gy * Gz;
+  const int mu = comp < 3 ? 0 : (comp < 5 ? 1 : 2);
+  const int nu = comp == 0 ? 0 : (comp == 1 || comp == 3) ? 1 : 2;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(t % Gx);
+    const int y = (int)((t / Gx) % Gy);
+    const int z = (int)(t / ((long long)Gx * Gy));
+    const int i = x <= Gx / 2 ? x : x - Gx;
+    const int j = y <= Gy / 2 ? y : y - Gy;
+    const int k = z <= Gz / 2 ? z : z - Gz;
+    double2 v = make_double2(0.0, 0.0);
+    if (abs(i) < cx && abs(j) < cy && abs(k) < cz && (i | j | k) != 0) {
+      const double r[3] = {(double)i, (double)j, (double)k};
+      const double rr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      const double kr = kd * rr;
+      const double kr2 = kr * kr;
+      const double qmn = (r[mu] / rr) * (r[nu] / rr);
+      double sn, cs;
+      sincos(kr, &sn, &cs);
+      const double inv3 = 1.0 / (rr * rr * rr);
+      // exp(i kr) / r^3 * [(3 - kr^2 - 3 i kr) q_mu q_nu + delta (kr^2 - 1 + i kr)]
+      double br = (3.0 - kr2) * qmn, bi = -3.0 * kr * qmn;
+      if (mu == nu) {
+        br += kr2 - 1.0;
+        bi += kr;
+      }
+      v.x = (cs * br - sn * bi) * inv3;
+      v.y = (cs * bi + sn * br) * inv3;
+    }
+    w[t] = v;
+  }
+}
+
+// Octant extraction with the -1/N of DDA/DMatrixCpu.cpp:200.
+template <typename R>
+__global__ void k_dhat_extract(const double2* __restrict__ w, typename CxT<R>::T* dhat, int comp, int Gx, int Gy,
+                               int Gz, double scale) {
+  typedef typename CxT<R>::T C;
+  const int hx = Gx / 2 + 1, hy = Gy / 2 + 1, hz = Gz / 2 + 1;
+  const long long total = (long long)hx * hy * hz;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+    const int fx = (int)(t % hx);
+    const int fy = (int)((t / hx) % hy);
+    const int fz = (int)(t / ((long long)hx * hy));
+    const double2 v = w[((long long)fz * Gy + fy) * Gx + fx];
+    dhat[t * 6 + comp] = cmake<C>(v.x * scale, v.y * scale);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------
+
+bool make_plan(int n, FftPlan* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->n = n;
+  int m = n, s = 0;
+  auto push = [&](int r) {
+    p->radix[s++] = r;
+    m /= r;
+  };
+  while (m % 8 == 0 && s < kMaxStages) push(8);
+  while (m % 4 == 0 && s < kMaxStages) push(4);
+  while (m % 2 == 0 && s < kMaxStages) push(2);
+  for (int r : {3, 5, 7})
+    while (m % r == 0 && s < kMaxStages) push(r);
+  if (m != 1) return false;
+  p->nst = s;
+  int ns = 1;
+  for (int i = 0; i < s; i++) {
+    p->ns[i] = ns;
+    p->nsdiv[i] = make_fastdiv(ns);
+    p->twmul[i] = n / (ns * p->radix[i]);
+    ns *= p->radix[i];
+  }
+  return true;
+}
+
+dda_status make_twiddles(int n, int prec, void** out) {
+  const long double pi = 3.1415926535897932384626433832795028L;
+  size_t esz = prec == 64 ? sizeof(double2) : sizeof(float2);
+  std::vector<char> h(esz * n);
+  for (int t = 0; t < n; t++) {
+    long double a = -2 * pi * (long double)t / (long double)n;
+    double c = (double)cosl(a), s = (double)sinl(a);
+    if (prec == 64)
+      reinterpret_cast<double2*>(h.data())[t] = make_double2(c, s);
+    else
+      reinterpret_cast<float2*>(h.data())[t] = make_float2((float)c, (float)s);
+  }
+  DDA_TRY_HIP(hipMalloc(out, h.size()));
+  DDA_TRY_HIP(hipMemcpy(*out, h.data(), h.size(), hipMemcpyHostToDevice));
+  return DDA_OK;
+}
+
+namespace {
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Threads per block for a tile of `pts` points with PPT points per thread.
+inline int block_for(int pts, int ppt) {
+  int b = round_up((pts + ppt - 1) / ppt, 64);
+  return b < 64 ? 64 : b;
+}
+
+// Columns per work-group: `pref`, halved until the tile fits 16 points per thread.
+inline int col_tile(int n, int pref) {
+  int t = pref;
+  while (t > 1 && n * t > 16 * 256) t /= 2;
+  return t;
+}
+
+template <typename K>
+dda_status set_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024) DDA_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return DDA_OK;
+}
+
+template <typename R, bool INV>
+dda_status launch_col(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
+                      int nbatch, hipStream_t st, const int* done) {
+  typedef typename CxT<R>::T C;
+  const int pts = p.n * T;
+  const size_t lds = (size_t)pts * sizeof(C);
+  dim3 grid((cp.ncols + T - 1) / T, nbatch);
+  FastDiv tdiv = make_fastdiv(T);
+  if (pts <= 8 * 256) {
+    auto k = k_colfft<R, 8, INV>;
+    DDA_TRY(set_lds(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, (const C*)in, (C*)out, cp, p, tdiv, (const C*)tw,
+                       done);
+  } else if (pts <= 16 * 256) {
+    auto k = k_colfft<R, 16, INV>;
+    DDA_TRY(set_lds(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, (const C*)in, (C*)out, cp, p, tdiv,
+                       (const C*)tw, done);
+  } else {
+    set_error("column FFT tile too large");
+    return DDA_NOT_SUPPORTED;
+  }
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
+template <typename R>
+dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void* tw, hipStream_t st) {
+  typedef typename CxT<R>::T C;
+  int L = (8 * 256) / p.n;
+  if (L < 1) L = 1;
+  int ppt = 8;
+  if (p.n * L > 8 * 256) ppt = 16;
+  const int pts = p.n * L;
+  if (pts > 16 * 256) {
+    set_error("row FFT too long");
+    return DDA_NOT_SUPPORTED;
+  }
+  const size_t lds = (size_t)pts * sizeof(C);
+  dim3 grid((unsigned)((nlines + L - 1) / L));
+  FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(p.n);
+  if (ppt == 8) {
+    auto k = k_rowfft<R, 8, false>;
+    DDA_TRY(set_lds(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, (C*)data, nlines, p, ldiv, ndiv, (const C*)tw);
+  } else {
+    auto k = k_rowfft<R, 16, false>;
+    DDA_TRY(set_lds(k, lds));
+    hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, (C*)data, nlines, p, ldiv, ndiv, (const C*)tw);
+  }
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
+}  // namespace
+
+// The five passes of one matvec on `st`. done (nullable) lets a device-side
+// solver turn the launches into no-ops once it has finished.
+template <typename R>
+dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done) {
+  typedef typename CxT<R>::T C;
+  const MvDims& d = op->dims;
+  const C* x = (const C*)xv;
+  C* y = (C*)yv;
+  C* a1 = (C*)op->a1;
+  C* a2 = (C*)op->a2;
+  const C* ccs = (const C*)op->ccs;
+
+  // pass 1 / pass 5 shapes
+  const int lr = op->p1_rows;
+  const int L = 3 * lr;
+  const int pts_x = d.Gx * L;
+  const int ppt_x = pts_x <= 8 * 256 ? 8 : 16;
+  const size_t lds_x = (size_t)pts_x * sizeof(C);
+  const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
+  {
+    dim3 grid((d.nrows + lr - 1) / lr);
+    if (ppt_x == 8) {
+      auto k = k_pass1<R, 8>;
+      DDA_TRY(set_lds(k, lds_x));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, x, a1, op->row_start, op->row_dip,
+                         op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+    } else {
+      auto k = k_pass1<R, 16>;
+      DDA_TRY(set_lds(k, lds_x));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, x, a1, op->row_start, op->row_dip,
+                         op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+    }
+    DDA_TRY_HIP(hipGetLastError());
+  }
+  // pass 2: Y forward, A1 -> A2, batch = (c, z)
+  {
+    ColParams cp;
+    cp.ncols = d.Gx;
+    cp.in_len = d.By;
+    cp.out_len = d.Gy;
+    cp.pstride_in = d.Gx;
+    cp.pstride_out = d.Gx;
+    cp.bstride_in = (long long)d.By * d.Gx;
+    cp.bstride_out = (long long)d.Gy * d.Gx;
+    DDA_TRY((launch_col<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+  }
+  // pass 3: Z forward, D multiply, Z inverse, in place on A2
+  {
+    const int T = op->col_t;
+    const int pts = d.Gz * T;
+    const size_t lds = (size_t)pts * sizeof(C);
+    dim3 grid((d.Gx + T - 1) / T, d.Gy);
+    FastDiv tdiv = make_fastdiv(T);
+    if (pts <= 8 * 256) {
+      auto k = k_pass3<R, 8>;
+      DDA_TRY(set_lds(k, lds));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
+                         (const C*)op->tw[2], done);
+    } else if (pts <= 16 * 256) {
+      auto k = k_pass3<R, 16>;
+      DDA_TRY(set_lds(k, lds));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
+                         (const C*)op->tw[2], done);
+    } else {
+      set_error("Z pass tile too large");
+      return DDA_NOT_SUPPORTED;
+    }
+    DDA_TRY_HIP(hipGetLastError());
+  }
+  // pass 4: Y inverse, A2 -> A1 cropped to y < By
+  {
+    ColParams cp;
+    cp.ncols = d.Gx;
+    cp.in_len = d.Gy;
+    cp.out_len = d.By;
+    cp.pstride_in = d.Gx;
+    cp.pstride_out = d.Gx;
+    cp.bstride_in = (long long)d.Gy * d.Gx;
+    cp.bstride_out = (long long)d.By * d.Gx;
+    DDA_TRY((launch_col<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+  }
+  // pass 5
+  {
+    dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1);
+    if (ppt_x == 8) {
+      auto k = k_pass5<R, 8>;
+      DDA_TRY(set_lds(k, lds_x));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
+                         op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
+                         (const C*)op->tw[0], conj, done);
+    } else {
+      auto k = k_pass5<R, 16>;
+      DDA_TRY(set_lds(k, lds_x));
+      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
+                         op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
+                         (const C*)op->tw[0], conj, done);
+    }
+    DDA_TRY_HIP(hipGetLastError());
+  }
+  return DDA_OK;
+}
+
+template dda_status launch_matvec<float>(dda_op_s*, const void*, void*, int, hipStream_t, const int*);
+template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hipStream_t, const int*);
+
+// Generate the octant of the FFT'd, d^3-scaled tensor on the device in FP64:
+// fill one component on the full grid, 3-D FFT (X rows, Y and Z columns), keep
+// the octant times -1/N, converted to the operator's precision.
+static dda_status generate_dhat(dda_op_s* op) {
+  const MvDims& d = op->dims;
+  hipStream_t st = op->ctx->stream;
+  const long long total = (long long)d.Gx * d.Gy * d.Gz;
+  double2* w = nullptr;
+  DDA_TRY_HIP(hipMalloc(&w, total * sizeof(double2)));
+  void* tw64[3] = {nullptr, nullptr, nullptr};
+  dda_status rc = DDA_OK;
+  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(op->plan[a].n, 64, &tw64[a]);
+  const double kd = op->wave_num * op->grid_unit;
+  const double scale = -1.0 / (double)total;
+  for (int comp = 0; comp < 6 && rc == DDA_OK; comp++) {
+    hipLaunchKernelGGL(k_green_fill, dim3(2048), dim3(256), 0, st, w, comp, d.Gx, d.Gy, d.Gz, d.Bx, d.By, d.Bz, kd);
+    rc = launch_row<double>(w, (long long)d.Gy * d.Gz, op->plan[0], tw64[0], st);
+    if (rc != DDA_OK) break;
+    ColParams cy;
+    cy.ncols = d.Gx;
+    cy.in_len = cy.out_len = d.Gy;
+    cy.pstride_in = cy.pstride_out = d.Gx;
+    cy.bstride_in = cy.bstride_out = (long long)d.Gx * d.Gy;
+    rc = launch_col<double, false>(w, w, cy, op->plan[1], col_tile(d.Gy, 8), tw64[1], d.Gz, st, nullptr);
+    if (rc != DDA_OK) break;
+    ColParams cz;
+    cz.ncols = d.Gx * d.Gy;
+    cz.in_len = cz.out_len = d.Gz;
+    cz.pstride_in = cz.pstride_out = (long long)d.Gx * d.Gy;
+    cz.bstride_in = cz.bstride_out = 0;
+    rc = launch_col<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, nullptr);
+    if (rc != DDA_OK) break;
+    if (op->prec == 64)
+      hipLaunchKernelGGL(k_dhat_extract<double>, dim3(1024), dim3(256), 0, st, w, (double2*)op->dhat, comp, d.Gx, d.Gy,
+                         d.Gz, scale);
+    else
+      hipLaunchKernelGGL(k_dhat_extract<float>, dim3(1024), dim3(256), 0, st, w, (float2*)op->dhat, comp, d.Gx, d.Gy,
+                         d.Gz, scale);
+    if (hipGetLastError() != hipSuccess) {
+      set_error("tensor generation launch failed");
+      rc = DDA_HIP_ERROR;
+    }
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  for (int a = 0; a < 3; a++)
+    if (tw64[a]) (void)hipFree(tw64[a]);
+  (void)hipFree(w);
+  if (rc != DDA_OK) return rc;
+  DDA_TRY_HIP(e);
+  return DDA_OK;
+}
+
+}  // namespace dda
+
+using namespace dda;
+
+extern "C" uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot) {
+  // DDA/DDAParams.cpp:45-90
+  if (override_ != 0) return x <= override_ ? override_ : 0;
+  if (x == 0) return 0;
+  if (!support_non_pot) {
+    uint32_t ret = 1;
+    while (x > 1) {
+      ret *= 2;
+      x = (x + 1) / 2;
+    }
+    return ret;
+  }
+  uint32_t ret = x + (x & 1);
+  for (;;) {
+    uint32_t y = ret;
+    for (uint32_t p : {2u, 3u, 5u, 7u})
+      while (y % p == 0) y /= p;
+    if (y == 1) return ret;
+    ret += 2;
+  }
+}
+
+static void free_op(dda_op_s* op) {
+  void* bufs[] = {op->row_start, op->row_dip, op->row_meta, op->row_list, op->mat_dev, op->ccs, op->dhat, op->a1,
+                  op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete op;
+}
+
+extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out) {
+  if (!ctx || !desc || !out || (precision != 32 && precision != 64)) {
+    set_error("dda_op_create: null argument or precision not 32/64");
+    return DDA_INVALID_ARG;
+  }
+  *out = nullptr;
+  const dda_op_desc& ds = *desc;
+  if (ds.nv == 0 || !ds.pos || ds.nmat == 0 || ds.nmat > 256 || !(ds.grid_unit > 0) || !(ds.wave_num > 0)) {
+    set_error("dda_op_create: need nv > 0, positions, 1..256 materials, grid_unit > 0, wave_num > 0");
+    return DDA_INVALID_ARG;
+  }
+  for (int a = 0; a < 3; a++)
+    if (ds.box[a] == 0 || ds.box[a] > 65535) {
+      set_error("dda_op_create: box extents must be in 1..65535");
+      return DDA_INVALID_ARG;
+    }
+  uint32_t grid[3];
+  for (int a = 0; a < 3; a++) {
+    grid[a] = ds.grid[a] ? ds.grid[a] : dda_fft_fit(2 * ds.box[a], 0, 1);
+    if (grid[a] < 2 * ds.box[a] || grid[a] % 2) {
+      set_error("dda_op_create: grid must be even and at least twice the box (DDA/DDAParams.cpp:45-50,131-133)");
+      return DDA_INVALID_ARG;
+    }
+  }
+  const uint32_t S = ds.vec_stride ? ds.vec_stride : (ds.nv + 15) / 16 * 16;
+  if (S < ds.nv) {
+    set_error("dda_op_create: vec_stride < nv");
+    return DDA_INVALID_ARG;
+  }
+  dda_op_s* op = new dda_op_s();
+  op->ctx = ctx;
+  op->prec = precision;
+  op->nmat = ds.nmat;
+  op->grid_unit = ds.grid_unit;
+  op->wave_num = ds.wave_num;
+  MvDims& d = op->dims;
+  d.Bx = ds.box[0]; d.By = ds.box[1]; d.Bz = ds.box[2];
+  d.Gx = grid[0]; d.Gy = grid[1]; d.Gz = grid[2];
+  d.nv = ds.nv;
+  d.S = S;
+  d.nrows = d.By * d.Bz;
+  for (int a = 0; a < 3; a++)
+    if (!make_plan(grid[a], &op->plan[a])) {
+      set_error("dda_op_create: grid sizes must factor into 2, 3, 5, 7");
+      free_op(op);
+      return DDA_NOT_SUPPORTED;
+    }
+  op->pos.assign(ds.pos, ds.pos + 3 * (size_t)ds.nv);
+  op->mat.assign(ds.nv, 0);
+  if (ds.mat) op->mat.assign(ds.mat, ds.mat + ds.nv);
+
+  // Rows of the (z, y) plane: counting sort of the dipoles by row, stable.
+  std::vector<uint32_t> row_start(d.nrows + 1, 0), row_dip(ds.nv), row_meta(ds.nv), row_list;
+  for (uint32_t i = 0; i < ds.nv; i++) {
+    const uint32_t* p = &op->pos[3 * i];
+    if ((int)p[0] >= d.Bx || (int)p[1] >= d.By || (int)p[2] >= d.Bz || op->mat[i] >= ds.nmat) {
+      set_error("dda_op_create: dipole position outside the box or material index >= nmat");
+      free_op(op);
+      return DDA_INVALID_ARG;
+    }
+    row_start[p[2] * d.By + p[1] + 1]++;
+  }
+  for (int r = 0; r < d.nrows; r++) {
+    if (row_start[r + 1]) row_list.push_back(r);
+    row_start[r + 1] += row_start[r];
+  }
+  {
+    std::vector<uint32_t> fill(row_start.begin(), row_start.end() - 1);
+    for (uint32_t i = 0; i < ds.nv; i++) {
+      const uint32_t* p = &op->pos[3 * i];
+      const uint32_t r = p[2] * d.By + p[1];
+      const uint32_t k = fill[r]++;
+      row_dip[k] = i;
+      row_meta[k] = p[0] | ((uint32_t)op->mat[i] << 16);
+    }
+  }
+  op->n_nonempty = (int)row_list.size();
+
+  // Launch shapes: X passes hold ~2048 points per work-group; column passes
+  // take 128-byte row segments (8 complex doubles, 16 complex floats).
+  const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
+  op->p1_rows = std::max(1, 2048 / (3 * d.Gx));
+  op->col_t = precision == 64 ? 8 : 16;
+  while (op->col_t > 1 && std::max(d.Gy, d.Gz) * op->col_t > 16 * 256) op->col_t /= 2;
+  if (3 * d.Gx > 16 * 256) {
+    set_error("dda_op_create: grid x extent too large for the single-tile X pass");
+    free_op(op);
+    return DDA_NOT_SUPPORTED;
+  }
+
+  auto dmalloc = [&](void** p, size_t bytes) -> dda_status {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      set_error(std::string("dda_op_create: hipMalloc failed: ") + hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+    }
+    op->device_bytes += bytes;
+    return DDA_OK;
+  };
+  const size_t a1n = 3ull * d.Bz * d.By * d.Gx;
+  const size_t a2n = 3ull * d.Bz * d.Gy * d.Gx;
+  const size_t dn = 6ull * (d.Gx / 2 + 1) * (d.Gy / 2 + 1) * (d.Gz / 2 + 1);
+  dda_status rc = DDA_OK;
+  if (rc == DDA_OK) rc = dmalloc((void**)&op->row_start, row_start.size() * 4);
+  if (rc == DDA_OK) rc = dmalloc((void**)&op->row_dip, ds.nv * 4ull);
+  if (rc == DDA_OK) rc = dmalloc((void**)&op->row_meta, ds.nv * 4ull);
+  if (rc == DDA_OK) rc = dmalloc((void**)&op->row_list, std::max<size_t>(1, row_list.size()) * 4);
+  if (rc == DDA_OK) rc = dmalloc((void**)&op->mat_dev, ds.nv);
+  if (rc == DDA_OK) rc = dmalloc(&op->ccs, 3ull * ds.nmat * csz);
+  if (rc == DDA_OK) rc = dmalloc(&op->dhat, dn * csz);
+  if (rc == DDA_OK) rc = dmalloc(&op->a1, a1n * csz);
+  if (rc == DDA_OK) rc = dmalloc(&op->a2, a2n * csz);
+  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(grid[a], precision, &op->tw[a]);
+  if (rc != DDA_OK) {
+    free_op(op);
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMemcpy(op->row_start, row_start.data(), row_start.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(op->row_dip, row_dip.data(), ds.nv * 4ull, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(op->row_meta, row_meta.data(), ds.nv * 4ull, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !row_list.empty())
+    e = hipMemcpy(op->row_list, row_list.data(), row_list.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(op->mat_dev, op->mat.data(), ds.nv, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(op->ccs, 0, 3ull * ds.nmat * csz);
+  if (e != hipSuccess) {
+    set_error(std::string("dda_op_create: upload failed: ") + hipGetErrorString(e));
+    free_op(op);
+    return DDA_HIP_ERROR;
+  }
+
+  if (ds.dmatrix) {
+    // Reference layout ((j + Gy*(k + Gz*i))*6 + comp), 1/m^3 -> octant, d^3 scaled.
+    const double d3 = ds.grid_unit * ds.grid_unit * ds.grid_unit;
+    const int hx = d.Gx / 2 + 1, hy = d.Gy / 2 + 1, hz = d.Gz / 2 + 1;
+    std::vector<char> h(dn * csz);
+    for (int fz = 0; fz < hz; fz++)
+      for (int fy = 0; fy < hy; fy++)
+        for (int fx = 0; fx < hx; fx++)
+          for (int c = 0; c < 6; c++) {
+            const size_t src = (((size_t)fy + (size_t)d.Gy * (fz + (size_t)d.Gz * fx)) * 6 + c) * 2;
+            const size_t dst = ((((size_t)fz * hy + fy) * hx + fx) * 6 + c);
+            const double re = ds.dmatrix[src] * d3, im = ds.dmatrix[src + 1] * d3;
+            if (precision == 64)
+              reinterpret_cast<double2*>(h.data())[dst] = make_double2(re, im);
+            else
+              reinterpret_cast<float2*>(h.data())[dst] = make_float2((float)re, (float)im);
+          }
+    e = hipMemcpy(op->dhat, h.data(), h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error(std::string("dda_op_create: tensor upload failed: ") + hipGetErrorString(e));
+      free_op(op);
+      return DDA_HIP_ERROR;
+    }
+  } else {
+    rc = generate_dhat(op);
+    if (rc != DDA_OK) {
+      free_op(op);
+      return rc;
+    }
+  }
+  *out = op;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_destroy(dda_op_t op) {
+  if (!op) return DDA_OK;
+  (void)hipStreamSynchronize(op->ctx->stream);
+  free_op(op);
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_info(dda_op_t op, uint32_t grid[3], uint32_t* vec_stride, size_t* device_bytes) {
+  if (!op) {
+    set_error("dda_op_info: null operator");
+    return DDA_INVALID_ARG;
+  }
+  if (grid) {
+    grid[0] = op->dims.Gx;
+    grid[1] = op->dims.Gy;
+    grid[2] = op->dims.Gz;
+  }
+  if (vec_stride) *vec_stride = op->dims.S;
+  if (device_bytes) *device_bytes = op->device_bytes;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_set_cc_sqrt(dda_op_t op, const double* cc_sqrt) {
+  if (!op || !cc_sqrt) {
+    set_error("dda_op_set_cc_sqrt: null argument");
+    return DDA_INVALID_ARG;
+  }
+  const double f = std::pow(op->grid_unit, -1.5);
+  const size_t n = 3ull * op->nmat;
+  op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 2 * n);
+  hipError_t e;
+  if (op->prec == 64) {
+    std::vector<double2> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_double2(cc_sqrt[2 * i] * f, cc_sqrt[2 * i + 1] * f);
+    e = hipMemcpyAsync(op->ccs, h.data(), n * sizeof(double2), hipMemcpyHostToDevice, op->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
+  } else {
+    std::vector<float2> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_float2((float)(cc_sqrt[2 * i] * f), (float)(cc_sqrt[2 * i + 1] * f));
+    e = hipMemcpyAsync(op->ccs, h.data(), n * sizeof(float2), hipMemcpyHostToDevice, op->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
+  }
+  DDA_TRY_HIP(e);
+  op->cc_set = true;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_apply(dda_op_t op, const void* x_dev, void* y_dev, int conj, dda_stream_t stream) {
+  if (!op || !x_dev || !y_dev || x_dev == y_dev) {
+    set_error("dda_op_apply: null or aliasing vectors");
+    return DDA_INVALID_ARG;
+  }
+  if (!op->cc_set) {
+    set_error("dda_op_apply: dda_op_set_cc_sqrt has not been called");
+    return DDA_INVALID_ARG;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : op->ctx->stream;
+  if (op->prec == 64) return launch_matvec<double>(op, x_dev, y_dev, conj, st, nullptr);
+  return launch_matvec<float>(op, x_dev, y_dev, conj, st, nullptr);
+}
+
+extern "C" dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj) {
+  if (!op || !x || !y) {
+    set_error("dda_op_apply_host: null argument");
+    return DDA_INVALID_ARG;
+  }
+  const size_t n = 3ull * op->dims.S;
+  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
+  if (!op->stage_x) {
+    DDA_TRY_HIP(hipMalloc(&op->stage_x, n * csz));
+    DDA_TRY_HIP(hipMalloc(&op->stage_y, n * csz));
+    op->device_bytes += 2 * n * csz;
+  }
+  hipStream_t st = op->ctx->stream;
+  if (op->prec == 64) {
+    DDA_TRY_HIP(hipMemcpyAsync(op->stage_x, x, n * csz, hipMemcpyHostToDevice, st));
+  } else {
+    std::vector<float2> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_float2((float)x[2 * i], (float)x[2 * i + 1]);
+    DDA_TRY_HIP(hipMemcpyAsync(op->stage_x, h.data(), n * csz, hipMemcpyHostToDevice, st));
+    DDA_TRY_HIP(hipStreamSynchronize(st));
+  }
+  DDA_TRY(dda_op_apply(op, op->stage_x, op->stage_y, conj, st));
+  if (op->prec == 64) {
+    DDA_TRY_HIP(hipMemcpyAsync(y, op->stage_y, n * csz, hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipStreamSynchronize(st));
+  } else {
+    std::vector<float2> h(n);
+    DDA_TRY_HIP(hipMemcpyAsync(h.data(), op->stage_y, n * csz, hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; i++) {
+      y[2 * i] = h[i].x;
+      y[2 * i + 1] = h[i].y;
+    }
+  }
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
+  if (!op || !out) {
+    set_error("dda_op_get_dmatrix: null argument");
+    return DDA_INVALID_ARG;
+  }
+  const MvDims& d = op->dims;
+  const int hx = d.Gx / 2 + 1, hy = d.Gy / 2 + 1, hz = d.Gz / 2 + 1;
+  const size_t dn = 6ull * hx * hy * hz;
+  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
+  std::vector<char> h(dn * csz);
+  DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  DDA_TRY_HIP(hipMemcpy(h.data(), op->dhat, dn * csz, hipMemcpyDeviceToHost));
+  const double inv_d3 = 1.0 / (op->grid_unit * op->grid_unit * op->grid_unit);
+  for (int i = 0; i < d.Gx; i++)
+    for (int k = 0; k < d.Gz; k++)
+      for (int j = 0; j < d.Gy; j++) {
+        const int fx = i <= d.Gx / 2 ? i : d.Gx - i;
+        const int fy = j <= d.Gy / 2 ? j : d.Gy - j;
+        const int fz = k <= d.Gz / 2 ? k : d.Gz - k;
+        const double sx = i <= d.Gx / 2 ? 1 : -1, sy = j <= d.Gy / 2 ? 1 : -1, sz = k <= d.Gz / 2 ? 1 : -1;
+        const double sgn[6] = {1, sx * sy, sx * sz, 1, sy * sz, 1};
+        for (int c = 0; c < 6; c++) {
+          const size_t src = (((size_t)fz * hy + fy) * hx + fx) * 6 + c;
+          double re, im;
+          if (op->prec == 64) {
+            double2 v = reinterpret_cast<double2*>(h.data())[src];
+            re = v.x;
+            im = v.y;
+          } else {
+            float2 v = reinterpret_cast<float2*>(h.data())[src];
+            re = v.x;
+            im = v.y;
+          }
+          const size_t dst = (((size_t)j + (size_t)d.Gy * (k + (size_t)d.Gz * i)) * 6 + c) * 2;
+          out[dst] = re * sgn[c] * inv_d3;
+          out[dst + 1] = im * sgn[c] * inv_d3;
+        }
+      }
+  return DDA_OK;
+}

@@ -1,0 +1,153 @@
+/*
+ * dda_hip.h - C ABI of the MI355X (gfx950) DDA FFT-convolution matvec and the
+ * Krylov solvers around it.
+ *
+ * This boundary replaces, for one GPU, the reference's OpenCL path:
+ *   GpuMatVec<T>              DDA/GpuMatVec.hpp:85-99, DDA/GpuMatVec.cpp:88-431
+ *   MatVecGpu<T> (host data)  DDA/MatVecGpu.cpp:51-68
+ *   MatVec<T>::apply          DDA/MatVec.hpp:54
+ *   DMatrixGpu::createDMatrix DDA/DMatrixGpu.cpp:32-41 (here generated on the device)
+ *   IterativeSolverBase<T>::getPolVec  DDA/IterativeSolverBase.cpp:40-103
+ *   GpuQmrCs / GpuBicgStab / GpuCgnr / GpuBicgCs  DDA/GpuQmrCs.hpp:81,
+ *                             DDA/GpuBicgStab.hpp:65, DDA/GpuCgnr.cpp:41, DDA/GpuBicgCs.cpp:41
+ *
+ * Conventions (as in the reference):
+ *   - complex numbers are interleaved (re, im); device vectors are complex
+ *     float (precision 32) or complex double (precision 64), host vectors are
+ *     always complex double;
+ *   - dipole vectors are SoA [3][vec_stride], vec_stride = roundUp(nv, 16)
+ *     (DDA/DDAParams.cpp:113), entries [nv, vec_stride) are zero on output;
+ *   - the operator is A x = x - sqrt(a) G sqrt(a) x (G(0) = 0), with the FFTs
+ *     unnormalised and the -1/N folded into the FFT'd interaction tensor
+ *     (DDA/DMatrixCpu.cpp:200); conj != 0 computes conj(A conj(x)) = A^H x;
+ *   - a solve takes the incident field and returns the dipole polarisation
+ *     p = sqrt(a) x (DDA/GpuIterativeSolver.cpp:136-144).
+ *
+ * Errors are status codes; dda_last_error() returns a thread-local message
+ * for the last failing call on that thread. The library owns all device
+ * memory it allocates; the caller owns host arrays, which are copied before a
+ * call returns. A handle is used by one host thread at a time; distinct
+ * handles may be used concurrently.
+ */
+#ifndef DDA_HIP_H
+#define DDA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dda_status {
+  DDA_OK = 0,
+  DDA_INVALID_ARG = 1,
+  DDA_OUT_OF_MEMORY = 2,
+  DDA_HIP_ERROR = 3,
+  DDA_BREAKDOWN = 4,   /* solver breakdown: the reference's ASSERTs, e.g. DDA/QmrCs.cpp:89-93 */
+  DDA_MAX_ITER = 5,    /* "Too many iterations", DDA/IterativeSolverBase.cpp:57-58 */
+  DDA_STAGNATION = 6,  /* "Too many iterations w/o increase", DDA/IterativeSolverBase.cpp:59-60 */
+  DDA_NOT_SUPPORTED = 7
+} dda_status;
+
+typedef struct dda_ctx_s* dda_ctx_t;
+typedef struct dda_op_s* dda_op_t;
+typedef struct dda_solver_s* dda_solver_t;
+typedef void* dda_stream_t; /* a hipStream_t; NULL selects the context's stream */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* dda_last_error(void);
+
+/* Context: one GPU and one in-order stream (DDA/DDAMain.cpp:556-575). */
+dda_status dda_ctx_create(int device, dda_ctx_t* out);
+dda_status dda_ctx_destroy(dda_ctx_t ctx);
+dda_status dda_ctx_synchronize(dda_ctx_t ctx);
+/* Device name and HBM capacity, for reports. */
+dda_status dda_ctx_info(dda_ctx_t ctx, char* name, size_t name_len, size_t* hbm_bytes, int* cu_count);
+
+/* Plain device memory for callers without an allocator of their own. */
+dda_status dda_malloc(dda_ctx_t ctx, size_t bytes, void** out);
+dda_status dda_free(dda_ctx_t ctx, void* ptr);
+dda_status dda_memcpy_h2d(dda_ctx_t ctx, void* dst, const void* src, size_t bytes);
+dda_status dda_memcpy_d2h(dda_ctx_t ctx, void* dst, const void* src, size_t bytes);
+
+/* FFT grid size for a box extent of x dipoles: fftFit, DDA/DDAParams.cpp:45-90.
+ * x is 2*box; override != 0 returns override (0 if it is smaller than x). */
+uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot);
+
+/* Operator description: the parts of DDAParams/DipoleGeometry the matvec reads
+ * (DDA/GpuMatVec.cpp:88-136). */
+typedef struct dda_op_desc {
+  uint32_t box[3];       /* dipole box (Bx, By, Bz) */
+  uint32_t grid[3];      /* FFT grid, even and >= 2*box; 0 = fftFit smooth rule */
+  uint32_t nv;           /* non-void dipoles */
+  uint32_t vec_stride;   /* 0 = roundUp(nv, 16) */
+  const uint32_t* pos;   /* host [nv][3] grid coordinates (DipoleGeometry::positions()) */
+  const uint8_t* mat;    /* host [nv] material index (DipoleGeometry::materialIndices) */
+  uint32_t nmat;         /* number of materials */
+  double grid_unit;      /* d, metres (DDAParams::gridUnit) */
+  double wave_num;       /* k, 1/metres (DDAParams::waveNum) */
+  /* Optional host D matrix in the reference layout (DDA/DMatrixCpu.cpp:133-205,
+   * SymMatrix3<complex<double>> [Gy][Gz][Gx] Fortran order, 1/m^3). NULL =
+   * generate it on the device in FP64 (DDA/DMatrixGpu.cpp:32-41 does it on the
+   * CPU with FFTW and uploads). */
+  const double* dmatrix;
+} dda_op_desc;
+
+/* GpuMatVec::create, DDA/GpuMatVec.hpp:85-86. precision is 32 or 64. */
+dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out);
+dda_status dda_op_destroy(dda_op_t op);
+/* Grid, vector stride and device bytes held by the operator. */
+dda_status dda_op_info(dda_op_t op, uint32_t grid[3], uint32_t* vec_stride, size_t* device_bytes);
+/* GpuMatVec::setCoupleConstants, DDA/GpuMatVec.cpp:139-142: sqrt of the
+ * polarisabilities, [nmat][3] complex double in m^(3/2). */
+dda_status dda_op_set_cc_sqrt(dda_op_t op, const double* cc_sqrt);
+/* GpuMatVec::apply, DDA/GpuMatVec.cpp:144-431: y = A x (or A^H x) on device
+ * vectors of the operator's precision, enqueued on `stream`. x and y must not
+ * alias. */
+dda_status dda_op_apply(dda_op_t op, const void* x_dev, void* y_dev, int conj, dda_stream_t stream);
+/* MatVecGpu::apply, DDA/MatVecGpu.cpp:51-68: host complex double in and out;
+ * synchronous. */
+dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);
+/* Copies the operator's FFT'd interaction tensor to the host in the reference
+ * layout (1/m^3), for parity checks of the device generation. */
+dda_status dda_op_get_dmatrix(dda_op_t op, double* out);
+
+typedef enum dda_solver_kind {
+  DDA_SOLVER_QMR_CS = 0,   /* DDA/GpuQmrCs.cpp */
+  DDA_SOLVER_BICGSTAB = 1, /* DDA/GpuBicgStab.cpp */
+  DDA_SOLVER_CGNR = 2,     /* DDA/GpuCgnr.cpp */
+  DDA_SOLVER_BICG_CS = 3   /* DDA/GpuBicgCs.cpp */
+} dda_solver_kind;
+
+typedef struct dda_solve_info {
+  uint64_t iterations;   /* Krylov iterations (summed over refinement steps) */
+  uint64_t matvecs;      /* operator applications */
+  uint64_t refinements;  /* outer FP64 residual evaluations (mixed mode), else 0 */
+  double rel_residual;   /* sqrt(residScale * inprodR) at exit, as the reference logs it */
+  double true_rel_residual; /* |b - A x| / |b| recomputed in FP64 after the solve */
+  double seconds;        /* wall time of the solve */
+} dda_solve_info;
+
+/* new GpuQmrCs(...) etc. (DDA/DDAMain.cpp:647-656). op must be an FP64
+ * operator. max_iter = 0 selects 3*Bx*By*Bz (DDA/DDAMain.cpp:640-644);
+ * max_res_increase = 0 selects the solver's reference default (QMR 50000,
+ * BiCGStab 30000, CGNR 10, BiCG 50000). */
+dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
+                             dda_solver_t* out);
+dda_status dda_solver_destroy(dda_solver_t s);
+/* IterativeSolverBase::getPolVec, DDA/IterativeSolverBase.cpp:40-103.
+ * einc: host [3*vec_stride] complex double; start (nullable) is a previous
+ * polarisation (--load-start-dip-pol, x0 = start / sqrt(a)); pol_out receives
+ * p = sqrt(a) x. Stops when |r|^2 < eps^2 |b|^2. */
+dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, const double* start,
+                            double* pol_out, dda_solve_info* info);
+/* residScale * |r|^2 after each iteration of the last solve (index 0 = start).
+ * Writes min(n, iterations + 1) values and returns that count in *written. */
+dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n, uint64_t* written);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDA_HIP_H */

@@ -1,0 +1,136 @@
+"""Parity of the HIP matvec with the oracle (FP64 <= 1e-12, FP32 <= 1e-5 relative).
+
+The oracle here is the long-double CPU restatement of MatVecCpu
+(DDA/MatVecCpu.cpp:67-150) with the D matrix of DDA/DMatrixCpu.cpp:133-205.
+"""
+import numpy as np
+import pytest
+
+import dda_amd as D
+from dda_amd import problem as P
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _ops(ctx, g, params, cs, precision, **kw):
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=precision,
+                     grid=params.grid, nmat=len(g.materials), **kw)
+    op.set_couple_constants(cs)
+    return op
+
+
+@pytest.mark.parametrize("name", cases.SMALL_CASES)
+@pytest.mark.parametrize("conj", [False, True])
+def test_matvec_fp64_matches_oracle(ctx, name, conj):
+    print("case", name, "conj", conj, flush=True)
+    g, params, cc, cs, prop, pol = cases.build(name)
+    orc = cases.oracle_problem(g, params, cs)
+    x = P.random_dipole_vector(params, seed=7)
+    y_ref = orc.matvec(x, conj=conj, precision=80)
+    op = _ops(ctx, g, params, cs, 64)
+    y = D.MatVecGpu(op).apply(x, conj=conj)
+    assert cases.rel_err(y, y_ref) <= FP64_TOL
+    assert np.all(y[params.nv:params.vec_stride] == 0)
+
+
+@pytest.mark.parametrize("name", cases.SMALL_CASES)
+def test_matvec_fp32_matches_oracle(ctx, name):
+    print("case", name, flush=True)
+    g, params, cc, cs, prop, pol = cases.build(name)
+    orc = cases.oracle_problem(g, params, cs)
+    x = P.random_dipole_vector(params, seed=8)
+    y_ref = orc.matvec(x, precision=80)
+    op = _ops(ctx, g, params, cs, 32)
+    y = D.MatVecGpu(op).apply(x)
+    assert cases.rel_err(y, y_ref) <= FP32_TOL
+
+
+@pytest.mark.parametrize("grid,pot", [((0, 0, 0), False), ((24, 20, 16), True), ((30, 18, 14), True)])
+def test_matvec_grid_choices(ctx, grid, pot):
+    """Power-of-two, larger and smooth grids give the same operator (exact embedding)."""
+    print("grid", grid, pot, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("ellipsoid_5_4_3", grid=grid, support_non_pot=pot)
+    orc = cases.oracle_problem(g, params, cs)
+    x = P.random_dipole_vector(params, seed=9)
+    y_ref = orc.matvec(x, precision=80)
+    y = D.MatVecGpu(_ops(ctx, g, params, cs, 64)).apply(x)
+    assert cases.rel_err(y, y_ref) <= FP64_TOL
+
+
+def test_dmatrix_device_generation_matches_oracle(ctx):
+    """FP64 tensor generated on the GPU vs DMatrixCpu::createDMatrix restated."""
+    import oracle as O
+    g, params, cc, cs, prop, pol = cases.build("box_10_8_6")
+    op = _ops(ctx, g, params, cs, 64)
+    dm = op.dmatrix()
+    ref = O.dmatrix(g.box, params.grid, g.grid_unit, params.wave_num)
+    assert np.max(np.abs(dm - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_host_dmatrix_input(ctx):
+    """A D matrix handed over in the reference layout gives the same operator."""
+    import oracle as O
+    g, params, cc, cs, prop, pol = cases.build("sphere_r4")
+    ref = O.dmatrix(g.box, params.grid, g.grid_unit, params.wave_num)
+    op = _ops(ctx, g, params, cs, 64, dmatrix=ref)
+    x = P.random_dipole_vector(params, seed=10)
+    y_ref = cases.oracle_problem(g, params, cs).matvec(x, precision=80)
+    y = D.MatVecGpu(op).apply(x)
+    assert cases.rel_err(y, y_ref) <= FP64_TOL
+
+
+def test_matvec_c1_full_size(ctx):
+    """C1: sphere 40 dipoles across (33,552 dipoles), 80^3 grid, FP64 and FP32."""
+    g, params, cc, cs, prop, pol = cases.build("sphere_c1")
+    assert params.nv == 33552 and params.grid == (80, 80, 80)
+    orc = cases.oracle_problem(g, params, cs)
+    x = P.random_dipole_vector(params)
+    y_ref = orc.matvec(x, precision=64)
+    y64 = D.MatVecGpu(_ops(ctx, g, params, cs, 64)).apply(x)
+    assert cases.rel_err(y64, y_ref) <= FP64_TOL
+    y32 = D.MatVecGpu(_ops(ctx, g, params, cs, 32)).apply(x)
+    assert cases.rel_err(y32, y_ref) <= FP32_TOL
+
+
+def test_matvec_complex_symmetry_large(ctx):
+    """Size-independent property at a 128^3 grid: x^T (A y) = y^T (A x)."""
+    g = P.sphere(32.0, 1.5 + 0.1j, 0.5e-6)
+    params = P.DDAParams(g, 6e-6)
+    prop, pol1, _ = P.beam_pols((0, 0, 1))
+    cc, cs = P.couple_constants(params, prop, pol1)
+    mv = D.MatVecGpu(_ops(ctx, g, params, cs, 64))
+    x = P.random_dipole_vector(params, seed=1)
+    y = P.random_dipole_vector(params, seed=2)
+    a = np.dot(x, mv.apply(y))
+    b = np.dot(y, mv.apply(x))
+    assert abs(a - b) <= 1e-12 * abs(a)
+    # A^H consistency: <A x, y> = <x, A^H y>
+    c = np.vdot(mv.apply(x), y)
+    d = np.vdot(x, mv.apply(y, conj=True))
+    assert abs(c - d) <= 1e-12 * abs(c)
+
+
+def test_device_apply_matches_host_apply(ctx):
+    g, params, cc, cs, prop, pol = cases.build("two_materials")
+    op = _ops(ctx, g, params, cs, 64)
+    x = P.random_dipole_vector(params, seed=3)
+    xd = D.DeviceVector(ctx, params.vec_size, 64)
+    yd = D.DeviceVector(ctx, params.vec_size, 64)
+    xd.upload(x)
+    op.apply(xd, yd)
+    ctx.synchronize()
+    np.testing.assert_array_equal(yd.download(), D.MatVecGpu(op).apply(x))
+
+
+def test_invalid_arguments(ctx):
+    g, params, cc, cs, prop, pol = cases.build("sphere_r4")
+    with pytest.raises(D.DDAError):
+        D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, grid=(8, 8, 8))  # grid < 2*box
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num)
+    with pytest.raises(D.DDAError):
+        D.MatVecGpu(op).apply(np.zeros(params.vec_size, complex))  # couple constants not set

@@ -1,0 +1,267 @@
+"""Pins the CPU oracle against what mathematics and library routines fix.
+
+Each check uses a different computation from the oracle's own: brute-force DFT
+and numpy.fft for the FFT, finite differences of the scalar Green function for
+the dyadic tensor, a dense direct sum for the FFT matvec, numpy.linalg.solve for
+the solvers, the optical theorem for Cext, and limits of Mie and LDR theory.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+@pytest.mark.parametrize("n", [8, 12, 14, 18, 30, 60, 80, 96, 128, 192, 256])
+def test_fft_against_brute_force_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)  # direct definition
+    ref = x @ W.T
+    f = O.fft(x)
+    assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(f - np.fft.fft(x))) <= 1e-13 * np.max(np.abs(ref))
+    # inverse is exp(+...), unnormalised: round trip gives n x
+    assert np.max(np.abs(O.fft(f, inverse=True) - n * x)) <= 1e-12 * n * np.max(np.abs(x))
+    # Parseval
+    assert abs(np.sum(np.abs(f) ** 2) - n * np.sum(np.abs(x) ** 2)) <= 1e-12 * n * np.sum(np.abs(x) ** 2)
+
+
+def test_fft_delta_and_constant():
+    n = 80
+    d = np.zeros(n, complex)
+    d[0] = 1
+    np.testing.assert_allclose(O.fft(d), np.ones(n), atol=1e-15)
+    np.testing.assert_allclose(O.fft(np.ones(n, complex)), d * n, atol=1e-12)
+
+
+def test_fft_fit_rules():
+    # smooth rule (DDAParams.cpp:63-89): even 2^a 3^b 5^c 7^d >= x
+    assert [O.fft_fit(x) for x in (80, 81, 256, 22, 26)] == [80, 84, 256, 24, 28]
+    # power-of-two rule (:54-61) and override (:46-49)
+    assert [O.fft_fit(x, 0, False) for x in (80, 96, 256, 257)] == [128, 128, 256, 512]
+    assert O.fft_fit(80, 96) == 96
+
+
+def _scalar_green(r, k):
+    rr = np.linalg.norm(r)
+    return np.exp(1j * k * rr) / rr
+
+
+def test_green_tensor_against_finite_differences():
+    """G = (k^2 I + grad grad) e^{ikr}/r, differentiated numerically."""
+    k = 0.7
+    h = 1e-4
+    for r in ([1.0, 0.0, 0.0], [1.0, 2.0, -1.5], [0.3, -0.4, 2.2]):
+        r = np.array(r)
+        for mu, nu in [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]:
+            e_mu = np.eye(3)[mu] * h
+            e_nu = np.eye(3)[nu] * h
+            d2 = (_scalar_green(r + e_mu + e_nu, k) - _scalar_green(r + e_mu - e_nu, k)
+                  - _scalar_green(r - e_mu + e_nu, k) + _scalar_green(r - e_mu - e_nu, k)) / (4 * h * h)
+            ref = d2 + (k * k * _scalar_green(r, k) if mu == nu else 0)
+            got = O.green(r, k, mu, nu)
+            assert abs(got - ref) <= 1e-6 * max(1.0, abs(ref)), (r, mu, nu, got, ref)
+    # static limit at r = (1,0,0): G_xx = 2, G_yy = -1
+    assert abs(O.green([1, 0, 0], 0.0, 0, 0) - 2) < 1e-15
+    assert abs(O.green([1, 0, 0], 0.0, 1, 1) + 1) < 1e-15
+
+
+def test_dmatrix_is_fft_of_clipped_green():
+    """DMatrixCpu::createDMatrix == numpy.fft.fftn of the clipped tensor / (-N)."""
+    box = np.array([4, 3, 2], np.uint32)
+    grid = np.array([8, 8, 6], np.uint32)
+    d, k = 1.0, 0.6
+    dm = O.dmatrix(box, grid, d, k)
+    comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    gx, gy, gz = (int(v) for v in grid)
+    bx, by, bz = (int(v) for v in box)
+    for c, (mu, nu) in enumerate(comps):
+        g = np.zeros((gx, gy, gz), complex)
+        for i in range(1 - bx, bx):
+            for j in range(1 - by, by):
+                for kk in range(1 - bz, bz):
+                    if i or j or kk:
+                        g[i % gx, j % gy, kk % gz] = O.green([i * d, j * d, kk * d], k, mu, nu)
+        ref = np.fft.fftn(g) / -(gx * gy * gz)
+        got = dm[:, :, :, c].transpose(0, 2, 1)  # [Gx, Gz, Gy] -> [Gx, Gy, Gz]
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def _problem(shape, p, d=0.5e-6, lam=6e-6, m=1.5 + 0.1j, pot=False, grid=None):
+    pos, box = O.geometry(shape, *p)
+    g = np.array([O.fft_fit(2 * int(b), 0, not pot) for b in box], np.uint32) if grid is None else np.array(grid)
+    prop, pol1, pol2 = O.beam_pols([0, 0, 1])
+    k = 2 * math.pi / lam
+    cc, cs = O.couple_constants("ldr", m, d, k * d, prop, pol1)
+    return O.Problem(pos, None, box, g, cs, d, k), prop, pol1, cc
+
+
+def _rand(P, seed=3):
+    rng = np.random.default_rng(seed)
+    x = np.zeros(3 * P.vec_stride, complex)
+    for c in range(3):
+        x[c * P.vec_stride: c * P.vec_stride + P.nv] = rng.uniform(-1, 1, P.nv) + 1j * rng.uniform(-1, 1, P.nv)
+    return x
+
+
+@pytest.mark.parametrize("shape,p", [("sphere", (3.0,)), ("box", (5, 4, 3)), ("ellipsoid", (4, 3, 2)),
+                                     ("cylinder", (6, 2.5))])
+@pytest.mark.parametrize("conj", [False, True])
+def test_matvec_against_direct_sum(shape, p, conj):
+    P, *_ = _problem(shape, p + (0.0,) * (3 - len(p)))
+    x = _rand(P)
+    y = P.matvec(x, conj=conj, precision=80)
+    y_ref = P.matvec_direct(x, conj=conj)
+    assert np.linalg.norm(y - y_ref) <= 1e-14 * np.linalg.norm(y_ref)
+    assert np.all(y[P.nv:P.vec_stride] == 0)
+
+
+def test_matvec_direct_sum_on_reference_geometry_fixture():
+    """The 480-dipole 10x8x6 box of DDA/box_10_8_6.geom."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "box_10_8_6.geom")
+    rows = [l.split() for l in open(path) if l.strip() and not l.startswith("#")]
+    pos = np.array(rows, np.uint32)[:, :3]
+    assert pos.shape == (480, 3)
+    box = pos.max(axis=0) + 1
+    d, k = 1.0, 0.45
+    prop, pol1, _ = O.beam_pols([0, 0, 1])
+    cc, cs = O.couple_constants("cm", 1.33 + 0.01j, d, k * d, prop, pol1)
+    P = O.Problem(pos, None, box, np.array([O.fft_fit(2 * int(b)) for b in box]), cs, d, k)
+    x = _rand(P, 5)
+    assert np.linalg.norm(P.matvec(x) - P.matvec_direct(x)) <= 1e-14 * np.linalg.norm(x)
+
+
+def test_matvec_grid_invariance_and_symmetry():
+    P1, *_ = _problem("sphere", (3.5, 0, 0))
+    P2, *_ = _problem("sphere", (3.5, 0, 0), pot=True)
+    P3, *_ = _problem("sphere", (3.5, 0, 0), grid=[30, 18, 16])
+    x, y = _rand(P1, 1), _rand(P1, 2)
+    a = P1.matvec(x)
+    assert np.linalg.norm(P2.matvec(x) - a) <= 1e-14 * np.linalg.norm(a)
+    assert np.linalg.norm(P3.matvec(x) - a) <= 1e-14 * np.linalg.norm(a)
+    # complex symmetry of A = I - sqrt(a) G sqrt(a): x^T A y = y^T A x
+    assert abs(np.dot(x, P1.matvec(y)) - np.dot(y, a)) <= 1e-13 * abs(np.dot(y, a))
+
+
+def _dense(P):
+    n = 3 * P.vec_stride
+    idx = [c * P.vec_stride + i for c in range(3) for i in range(P.nv)]
+    A = np.zeros((len(idx), len(idx)), complex)
+    for j, col in enumerate(idx):
+        e = np.zeros(n, complex)
+        e[col] = 1
+        A[:, j] = P.matvec(e)[idx]
+    return A, idx
+
+
+@pytest.mark.parametrize("kind", ["qmr", "bicgstab", "cgnr", "bicg"])
+def test_solvers_against_dense_lu(kind):
+    P, prop, pol1, cc = _problem("sphere", (2.5, 0, 0))
+    A, idx = _dense(P)
+    e = O.einc_plane(P.pos, P.box, P.vec_stride, P.grid_unit, P.wave_num, prop, pol1)
+    b = np.zeros_like(e)
+    for c in range(3):
+        s = slice(c * P.vec_stride, c * P.vec_stride + P.nv)
+        b[s] = P.cc_sqrt[0, c] * e[s]
+    x_ref = np.linalg.solve(A, b[idx])
+    st, pvec, it, hist = P.solve(kind, e, 1e-12, max_iter=3000)
+    assert st == O.ST_OK
+    p_ref = np.zeros_like(e)
+    p_ref[idx] = x_ref
+    for c in range(3):
+        s = slice(c * P.vec_stride, c * P.vec_stride + P.nv)
+        p_ref[s] *= P.cc_sqrt[0, c]
+    assert np.linalg.norm(pvec - p_ref) <= 1e-9 * np.linalg.norm(p_ref)
+    assert hist[-1] < 1e-24
+
+
+def test_solver_error_statuses():
+    P, prop, pol1, cc = _problem("sphere", (3.0, 0, 0))
+    e = O.einc_plane(P.pos, P.box, P.vec_stride, P.grid_unit, P.wave_num, prop, pol1)
+    st, _, it, _ = P.solve("qmr", e, 1e-14, max_iter=3)
+    assert st == O.ST_MAXITER and it == 3
+
+
+def test_optical_theorem_and_mie():
+    """Cext from sum Im(p.E*) equals Cext from the forward amplitude, and both
+    approach the Mie value as the sphere is resolved."""
+    lam = 6e-6
+    k = 2 * math.pi / lam
+    m = 1.5 + 0.1j
+    radius = 1e-6
+    errs = []
+    for nd in (4, 8):
+        d = radius / nd
+        P, prop, pol1, cc = _problem("sphere", (float(nd), 0, 0), d=d, lam=lam, m=m)
+        e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol1)
+        st, pvec, it, _ = P.solve("qmr", e, 1e-12, max_iter=2000)
+        assert st == 0
+        c1 = P.ext_cross_sum(pvec, e)
+        c2 = P.ext_cross_forward(pvec, prop, pol1)
+        assert abs(c1 - c2) <= 1e-9 * abs(c1)
+        # volume-equivalent radius of the dipole set
+        a_eq = (3 * P.nv / (4 * math.pi)) ** (1 / 3) * d
+        qext, qsca, qback, gsca = O.mie(k * a_eq, m)
+        errs.append(abs(c1 - qext * math.pi * a_eq ** 2) / (qext * math.pi * a_eq ** 2))
+    assert errs[-1] < 0.05
+    # Cext = Cabs + Csca with Csca from the far field on a (theta, phi) grid
+    nt, nph = 61, 48
+    th = np.linspace(0, math.pi, nt)
+    J = P.jones(pvec, None, True, prop, pol1, O.beam_pols(prop)[2], nt, 0.0, th[1] - th[0], nph, 0.0,
+                2 * math.pi / nph)
+    assert np.all(np.isfinite(J))
+
+
+def test_mie_limits():
+    # Rayleigh limit: Qabs ~ 4 x Im((m^2-1)/(m^2+2)), Qsca ~ 8/3 x^4 |(m^2-1)/(m^2+2)|^2
+    m = 1.5 + 0.1j
+    x = 1e-3
+    a = (m * m - 1) / (m * m + 2)
+    qext, qsca, qback, g = O.mie(x, m)
+    assert abs((qext - qsca) / (4 * x * a.imag) - 1) < 1e-5
+    assert abs(qsca / (8.0 / 3 * x ** 4 * abs(a) ** 2) - 1) < 1e-4
+    # no absorption: Qext == Qsca
+    qext, qsca, qback, g = O.mie(3.0, 1.33)
+    assert abs(qext - qsca) < 1e-12 * qext
+    # extinction paradox: Qext -> 2 for large x
+    qext, *_ = O.mie(20000.0, 1.33 + 0.01j)
+    assert abs(qext - 2) < 0.005
+    # Bohren & Huffman's BHMIE sample case (sphere radius 0.525 um, wavelength
+    # 0.6328 um, m = 1.55): QEXT = QSCA = 3.10543, QBACK = 2.92534, g = 0.63314
+    qext, qsca, qback, g = O.mie(2 * math.pi * 0.525 / 0.6328, 1.55)
+    assert abs(qext / 3.10543 - 1) < 2e-5 and abs(qsca / 3.10543 - 1) < 2e-5
+    assert abs(qback / 2.92534 - 1) < 2e-5 and abs(g / 0.63314 - 1) < 2e-5
+
+
+def test_ldr_limits():
+    d = 1.0
+    prop, pol1, _ = O.beam_pols([0, 0, 1])
+    m = 1.7
+    cm, _ = O.couple_constants("cm", m, d, 1e-3, prop, pol1)
+    ldr, _ = O.couple_constants("ldr", m, d, 1e-3, prop, pol1)
+    assert abs(ldr[0, 0] / cm[0, 0] - 1) < 1e-5  # LDR -> CM as kd -> 0
+    # radiative reaction: for real m, Im(1/a_LDR) - Im(1/a_CM) = -(2/3) k^3
+    kd = 0.4
+    cm, _ = O.couple_constants("cm", m, d, kd, prop, pol1)
+    ldr, _ = O.couple_constants("ldr", m, d, kd, prop, pol1)
+    assert abs((1 / ldr[0, 0]).imag - (1 / cm[0, 0]).imag + 2.0 / 3 * kd ** 3) < 1e-12
+    # CM: a / V -> 3/(4 pi) as m -> infinity, and 0 at m = 1
+    cmi, _ = O.couple_constants("cm", 1e6, d, kd, prop, pol1)
+    assert abs(cmi[0, 0] - 3 / (4 * math.pi)) < 1e-9
+    one, _ = O.couple_constants("cm", 1.0, d, kd, prop, pol1)
+    assert abs(one[0, 0]) == 0
+
+
+def test_geometry_counts():
+    # SURVEY figures: 40 across -> 33,552 dipoles; 128 across -> 1,099,136
+    pos, box = O.geometry("sphere", 20.0)
+    assert len(pos) == 33552 and list(box) == [40, 40, 40]
+    pos, box = O.geometry("box", 10, 8, 6)
+    assert len(pos) == 480
+    # sorted by (z, y, x)
+    key = pos[:, 2].astype(np.int64) * 10 ** 8 + pos[:, 1] * 10 ** 4 + pos[:, 0]
+    assert np.all(np.diff(key) > 0)
