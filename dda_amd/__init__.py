@@ -33,7 +33,7 @@ EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_fft_fit",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_apply",
-    "dda_op_apply_host", "dda_op_get_dmatrix",
+    "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
 ]
 
@@ -96,6 +96,8 @@ def lib():
         L.dda_op_apply.argtypes = [vp, vp, vp, i32, vp]
         L.dda_op_apply_host.argtypes = [vp, dp, dp, i32]
         L.dda_op_get_dmatrix.argtypes = [vp, dp]
+        L.dda_op_profile.argtypes = [vp, i32]
+        L.dda_op_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_create.argtypes = [vp, i32, u64, u64, pp]
         L.dda_solver_destroy.argtypes = [vp]
         L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
@@ -285,6 +287,18 @@ class GpuMatVec:
         out = np.zeros((gx, gz, gy, 6), np.complex128)
         _check(lib().dda_op_get_dmatrix(self.handle, _dp(out.view(np.float64))))
         return out
+
+    def profile(self, enable=True):
+        """Time each of the five kernels with HIP events on their own stream."""
+        _check(lib().dda_op_profile(self.handle, int(enable)))
+
+    def profile_read(self):
+        """-> (ms per pass [5], launches per pass [5], skipped no-op launches)."""
+        ms = np.zeros(5)
+        cnt = (ctypes.c_uint64 * 5)()
+        sk = ctypes.c_uint64()
+        _check(lib().dda_op_profile_read(self.handle, _dp(ms), cnt, ctypes.byref(sk)))
+        return ms, np.array(list(cnt), np.int64), int(sk.value)
 
     def close(self):
         if getattr(self, "handle", None):

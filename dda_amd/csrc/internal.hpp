@@ -88,6 +88,12 @@ struct dda_op_s {
   void* stage_y = nullptr;
   size_t device_bytes = 0;
 
+  // per-pass event timing (dda_op_profile)
+  bool prof = false;
+  struct ProfRec { int pass; hipEvent_t b, e; };
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
+
   // launch shapes
   int p1_rows = 1;   // rows per work-group in the X passes
   int col_t = 8;     // columns per work-group in the Y/Z passes

@@ -27,7 +27,7 @@ namespace dda {
 
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
-constexpr int kChunk = 8;  // iterations enqueued between flag reads
+constexpr int kChunk = 16;  // most iterations enqueued between flag reads
 
 struct SolverState {
   int done;     // 1 once the solve has finished for any reason
@@ -846,16 +846,36 @@ static dda_status run_solve(dda_solver_s* s, double eps, bool have_start, dda_so
   // Enqueue chunks of iterations; chunk c+1 is queued before waiting on the
   // state copied after chunk c, so the GPU never idles on the host. Chunks
   // queued after the device has finished run as no-ops.
+  // The chunk length follows the observed convergence rate so that few
+  // iterations are queued past the one that meets the tolerance.
   DDA_TRY_HIP(hipMemcpyAsync(&s->st_host[0], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
   DDA_TRY_HIP(hipEventRecord(ev[0], st));
+  int next_len = 2;
+  double prev_r = -1;
+  uint64_t prev_count = 0;
   while (!finished) {
-    for (int k = 0; k < kChunk; k++) DDA_TRY(enqueue_iteration());
+    for (int k = 0; k < next_len; k++) DDA_TRY(enqueue_iteration());
     const int slot = (chunk + 1) & 1;
     DDA_TRY_HIP(hipMemcpyAsync(&s->st_host[slot], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
     DDA_TRY_HIP(hipEventRecord(ev[slot], st));
     const int prev = chunk & 1;
     DDA_TRY_HIP(hipEventSynchronize(ev[prev]));
-    if (s->st_host[prev].done) finished = true;
+    const SolverState& hs = s->st_host[prev];
+    if (hs.done) {
+      finished = true;
+    } else {
+      // iterations still needed at the rate seen since the last read, less
+      // the chunk already queued behind this state
+      int len = kChunk;
+      if (prev_r > 0 && hs.inprodR > 0 && hs.count > prev_count && hs.inprodR < prev_r && hs.epsB > 0) {
+        const double rate = std::log(prev_r / hs.inprodR) / (double)(hs.count - prev_count);
+        const double need = std::log(hs.inprodR / hs.epsB) / rate - (double)next_len;
+        len = need < 1 ? 1 : (need > kChunk ? kChunk : (int)std::ceil(need));
+      }
+      prev_r = hs.inprodR;
+      prev_count = hs.count;
+      next_len = len;
+    }
     chunk++;
   }
   DDA_TRY_HIP(hipStreamSynchronize(st));

@@ -478,6 +478,38 @@ dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void
 
 // The five passes of one matvec on `st`. done (nullable) lets a device-side
 // solver turn the launches into no-ops once it has finished.
+static hipEvent_t pool_event(dda_op_s* op) {
+  if (!op->prof_pool.empty()) {
+    hipEvent_t e = op->prof_pool.back();
+    op->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets pass `pass` with events when profiling is on.
+struct PassTimer {
+  dda_op_s* op;
+  hipStream_t st;
+  int pass;
+  hipEvent_t b = nullptr, e = nullptr;
+  PassTimer(dda_op_s* o, hipStream_t s, int p) : op(o), st(s), pass(p) {
+    if (op->prof) {
+      b = pool_event(op);
+      e = pool_event(op);
+      if (b) (void)hipEventRecord(b, st);
+    }
+  }
+  ~PassTimer() {
+    if (b && e) {
+      (void)hipEventRecord(e, st);
+      op->prof_recs.push_back({pass, b, e});
+    }
+  }
+};
+
 template <typename R>
 dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done) {
   typedef typename CxT<R>::T C;
@@ -496,6 +528,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   const size_t lds_x = (size_t)pts_x * sizeof(C);
   const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
   {
+    PassTimer pt(op, st, 0);
     dim3 grid((d.nrows + lr - 1) / lr);
     if (ppt_x == 8) {
       auto k = k_pass1<R, 8>;
@@ -512,6 +545,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   }
   // pass 2: Y forward, A1 -> A2, batch = (c, z)
   {
+    PassTimer pt(op, st, 1);
     ColParams cp;
     cp.ncols = d.Gx;
     cp.in_len = d.By;
@@ -524,6 +558,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
   {
+    PassTimer pt(op, st, 2);
     const int T = op->col_t;
     const int pts = d.Gz * T;
     const size_t lds = (size_t)pts * sizeof(C);
@@ -547,6 +582,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   }
   // pass 4: Y inverse, A2 -> A1 cropped to y < By
   {
+    PassTimer pt(op, st, 3);
     ColParams cp;
     cp.ncols = d.Gx;
     cp.in_len = d.Gy;
@@ -559,6 +595,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   }
   // pass 5
   {
+    PassTimer pt(op, st, 4);
     dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1);
     if (ppt_x == 8) {
       auto k = k_pass5<R, 8>;
@@ -660,6 +697,11 @@ extern "C" uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_
 }
 
 static void free_op(dda_op_s* op) {
+  for (auto& r : op->prof_recs) {
+    (void)hipEventDestroy(r.b);
+    (void)hipEventDestroy(r.e);
+  }
+  for (auto e : op->prof_pool) (void)hipEventDestroy(e);
   void* bufs[] = {op->row_start, op->row_dip, op->row_meta, op->row_list, op->mat_dev, op->ccs, op->dhat, op->a1,
                   op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y};
   for (void* b : bufs)
@@ -968,5 +1010,42 @@ extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
           out[dst + 1] = im * sgn[c] * inv_d3;
         }
       }
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_profile(dda_op_t op, int enable) {
+  if (!op) {
+    set_error("dda_op_profile: null operator");
+    return DDA_INVALID_ARG;
+  }
+  op->prof = enable != 0;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_profile_read(dda_op_t op, double ms[5], uint64_t counts[5], uint64_t* skipped) {
+  if (!op || !ms || !counts) {
+    set_error("dda_op_profile_read: null argument");
+    return DDA_INVALID_ARG;
+  }
+  for (int i = 0; i < 5; i++) {
+    ms[i] = 0;
+    counts[i] = 0;
+  }
+  uint64_t sk = 0;
+  for (auto& r : op->prof_recs) {
+    DDA_TRY_HIP(hipEventSynchronize(r.e));
+    float t = 0;
+    DDA_TRY_HIP(hipEventElapsedTime(&t, r.b, r.e));
+    if (t < 0.002f) {
+      sk++;
+    } else {
+      ms[r.pass] += t;
+      counts[r.pass]++;
+    }
+    op->prof_pool.push_back(r.b);
+    op->prof_pool.push_back(r.e);
+  }
+  op->prof_recs.clear();
+  if (skipped) *skipped = sk;
   return DDA_OK;
 }

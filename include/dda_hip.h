@@ -112,6 +112,15 @@ dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);
 /* Copies the operator's FFT'd interaction tensor to the host in the reference
  * layout (1/m^3), for parity checks of the device generation. */
 dda_status dda_op_get_dmatrix(dda_op_t op, double* out);
+/* Kernel timing, in place of the reference's --sync per-stage profiling
+ * (DDA/GpuMatVec.cpp:149-164): while enabled, every matvec of this operator
+ * records HIP events around each of its five kernels on the stream it runs
+ * on. dda_op_profile_read synchronises, returns the summed durations (ms) and
+ * launch counts per pass since the last read, and clears the record. Launches
+ * that a finished solver turned into no-ops (shorter than 2 us) are left out
+ * of both and counted in *skipped. */
+dda_status dda_op_profile(dda_op_t op, int enable);
+dda_status dda_op_profile_read(dda_op_t op, double ms[5], uint64_t counts[5], uint64_t* skipped);
 
 typedef enum dda_solver_kind {
   DDA_SOLVER_QMR_CS = 0,   /* DDA/GpuQmrCs.cpp */
