@@ -1,7 +1,8 @@
 // Batched Stockham FFT through LDS for gfx950.
 //
 // A work-group holds a tile of L lines of length N in LDS, point-major:
-// tile[p * L + l]. Each radix-r stage (sub-length Ns = product of the earlier
+// tile[p * LP + l], with the point stride LP >= L padded against LDS bank
+// conflicts. Each radix-r stage (sub-length Ns = product of the earlier
 // radices) reads the r inputs j + m*N/r of butterfly j, twiddles them by
 // W_{Ns r}^{m k} (k = j mod Ns), applies an r-point DFT and writes them to
 // (j / Ns) * Ns * r + k + m * Ns, which leaves the result in natural order after
@@ -158,10 +159,11 @@ template <bool INV, typename C> struct Dft<5, INV, C> : DftOdd<5, INV, C> {};
 template <bool INV, typename C> struct Dft<7, INV, C> : DftOdd<7, INV, C> {};
 
 // One Stockham stage over `total` = N * L points of the tile. PPT bounds the
-// points per thread: total <= PPT * blockDim.x.
+// points per thread: total <= PPT * blockDim.x. The twiddles W^(m k) of a
+// butterfly come from one table load, W^k, and its powers.
 template <int RAD, int PPT, bool INV, typename C>
-__device__ inline void fft_stage(C* tile, int total, int nr /* N / RAD */, FastDiv ldiv, int ns, FastDiv nsdiv,
-                                 int twmul, const C* __restrict__ tw) {
+__device__ inline void fft_stage(C* tile, int total, int nr /* N / RAD */, FastDiv ldiv, int lp, int ns,
+                                 FastDiv nsdiv, int twmul, const C* __restrict__ tw) {
   constexpr int NB = (PPT + RAD - 1) / RAD;
   const int nbf = total / RAD;
   const int L = (int)ldiv.d;
@@ -173,7 +175,7 @@ __device__ inline void fft_stage(C* tile, int total, int nr /* N / RAD */, FastD
       const int j = ldiv.div(b);
       const int l = b - j * L;
 #pragma unroll
-      for (int m = 0; m < RAD; m++) v[i][m] = tile[(j + m * nr) * L + l];
+      for (int m = 0; m < RAD; m++) v[i][m] = tile[(j + m * nr) * lp + l];
     }
   }
   __syncthreads();
@@ -186,38 +188,45 @@ __device__ inline void fft_stage(C* tile, int total, int nr /* N / RAD */, FastD
       const int q = nsdiv.div(j);
       const int k = j - q * ns;
       if (ns > 1) {
+        const C w1 = tw[k * twmul];
+        C w = w1;
 #pragma unroll
         for (int m = 1; m < RAD; m++) {
-          C w = tw[m * k * twmul];
           v[i][m] = INV ? cmulc(v[i][m], w) : cmul(v[i][m], w);
+          if (m + 1 < RAD) w = cmul(w, w1);
         }
       }
       Dft<RAD, INV, C>::run(v[i]);
       const int base = q * ns * RAD + k;
 #pragma unroll
-      for (int m = 0; m < RAD; m++) tile[(base + m * ns) * L + l] = v[i][m];
+      for (int m = 0; m < RAD; m++) tile[(base + m * ns) * lp + l] = v[i][m];
     }
   }
   __syncthreads();
 }
 
-// Full transform of the tile. Callers must __syncthreads() after filling it.
+// Full transform of the tile (point stride lp). Callers must __syncthreads()
+// after filling it.
 template <int PPT, bool INV, typename C>
-__device__ inline void fft_tile(C* tile, const FftPlan& p, FastDiv ldiv, const C* __restrict__ tw) {
+__device__ inline void fft_tile(C* tile, const FftPlan& p, FastDiv ldiv, int lp, const C* __restrict__ tw) {
   const int total = p.n * (int)ldiv.d;
   for (int s = 0; s < p.nst; s++) {
     const int r = p.radix[s];
     const int nr = p.n / r;
     switch (r) {
-      case 8: fft_stage<8, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
-      case 4: fft_stage<4, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
-      case 2: fft_stage<2, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
-      case 3: fft_stage<3, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
-      case 5: fft_stage<5, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
-      case 7: fft_stage<7, PPT, INV>(tile, total, nr, ldiv, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 8: fft_stage<8, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 4: fft_stage<4, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 2: fft_stage<2, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 3: fft_stage<3, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 5: fft_stage<5, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
+      case 7: fft_stage<7, PPT, INV>(tile, total, nr, ldiv, lp, p.ns[s], p.nsdiv[s], p.twmul[s], tw); break;
       default: break;
     }
   }
 }
+
+// Point stride of an LDS tile with L lines: L plus one element when L is even,
+// so that consecutive points do not start on the same bank.
+__host__ __device__ inline int lds_stride(int L) { return (L & 1) ? L : L + 1; }
 
 }  // namespace dda

@@ -79,7 +79,7 @@ struct dda_op_s {
   uint8_t* mat_dev = nullptr;     // [nv] material index per dipole
   int n_nonempty = 0;
   void* ccs = nullptr;            // [nmat][3] complex R, d^(-3/2) scaled
-  void* dhat = nullptr;           // octant [Gz/2+1][Gy/2+1][Gx/2+1][6] complex R, d^3 scaled
+  void* dhat = nullptr;           // octant [6][Gz/2+1][Gy/2+1][Gx/2+1] complex R, d^3 scaled
   void* a1 = nullptr;             // [3][Bz][By][Gx]
   void* a2 = nullptr;             // [3][Bz][Gy][Gx]
   void* tw[3] = {nullptr, nullptr, nullptr};
@@ -97,4 +97,5 @@ struct dda_op_s {
   // launch shapes
   int p1_rows = 1;   // rows per work-group in the X passes
   int col_t = 8;     // columns per work-group in the Y/Z passes
+  bool static_z = false;  // pass 3 uses a compile-time planned kernel
 };

@@ -21,8 +21,10 @@
 // y and z), so D(G - k) = +-D(k) per axis (DDA/DMatrixCpu.cpp:49-63).
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
+#include "fft_static.hpp"
 #include "internal.hpp"
 
 namespace dda {
@@ -50,10 +52,11 @@ __global__ __launch_bounds__(256) void k_pass1(const typename CxT<R>::T* __restr
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
+  const int lp = lds_stride(L);
   const int N = d.Gx;
   const int pts = N * L;
   const int r0 = blockIdx.x * lr_n;
-  for (int i = threadIdx.x; i < pts; i += blockDim.x) tile[i] = czero<C>();
+  for (int i = threadIdx.x; i < N * lp; i += blockDim.x) tile[i] = czero<C>();
   __syncthreads();
   for (int lr = 0; lr < lr_n; lr++) {
     const int row = r0 + lr;
@@ -69,19 +72,19 @@ __global__ __launch_bounds__(256) void k_pass1(const typename CxT<R>::T* __restr
       for (int c = 0; c < 3; c++) {
         C v = x[(size_t)c * d.S + i];
         if (conj) v = cconj(v);
-        tile[xp * L + c * lr_n + lr] = cmul(ccs[3 * m + c], v);
+        tile[xp * lp + c * lr_n + lr] = cmul(ccs[3 * m + c], v);
       }
     }
   }
   __syncthreads();
-  fft_tile<PPT, false>(tile, px, ldiv, tw);
+  fft_tile<PPT, false>(tile, px, ldiv, lp, tw);
   for (int i = threadIdx.x; i < pts; i += blockDim.x) {
     const int l = ndiv.div(i);
     const int p = i - l * N;
     const int c = l / lr_n;
     const int lr = l - c * lr_n;
     const int row = r0 + lr;
-    if (row < d.nrows) a1[((size_t)c * d.nrows + row) * N + p] = tile[p * L + l];
+    if (row < d.nrows) a1[((size_t)c * d.nrows + row) * N + p] = tile[p * lp + l];
   }
 }
 
@@ -103,6 +106,7 @@ __global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restr
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
+  const int lp = lds_stride(L);
   const int N = d.Gx;
   const int pts = N * L;
   const int s_base = blockIdx.x * lr_n;
@@ -114,10 +118,10 @@ __global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restr
     const int slot = s_base + lr;
     C v = czero<C>();
     if (slot < n_list) v = a1[((size_t)c * d.nrows + row_list[slot]) * N + p];
-    tile[p * L + l] = v;
+    tile[p * lp + l] = v;
   }
   __syncthreads();
-  fft_tile<PPT, true>(tile, px, ldiv, tw);
+  fft_tile<PPT, true>(tile, px, ldiv, lp, tw);
   for (int lr = 0; lr < lr_n; lr++) {
     const int slot = s_base + lr;
     if (slot >= n_list) break;
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restr
       for (int c = 0; c < 3; c++) {
         C a = x[(size_t)c * d.S + i];
         if (conj) a = cconj(a);
-        C r = cadd(cmul(ccs[3 * m + c], tile[xp * L + c * lr_n + lr]), a);
+        C r = cadd(cmul(ccs[3 * m + c], tile[xp * lp + c * lr_n + lr]), a);
         if (conj) r = cconj(r);
         y[(size_t)c * d.S + i] = r;
       }
@@ -171,63 +175,93 @@ __global__ __launch_bounds__(256) void k_colfft(const typename CxT<R>::T* in, ty
   const long long b = blockIdx.y;
   const C* src = in + b * cp.bstride_in;
   C* dst = out + b * cp.bstride_out;
+  const int lp = lds_stride(T);
   const int pts = p.n * T;
   for (int i = threadIdx.x; i < pts; i += blockDim.x) {
     const int pp = tdiv.div(i);
-    const int col = c0 + i - pp * T;
+    const int l = i - pp * T;
+    const int col = c0 + l;
     C v = czero<C>();
     if (pp < cp.in_len && col < cp.ncols) v = src[pp * cp.pstride_in + col];
-    tile[i] = v;
+    tile[pp * lp + l] = v;
   }
   __syncthreads();
-  fft_tile<PPT, INV>(tile, p, tdiv, tw);
+  fft_tile<PPT, INV>(tile, p, tdiv, lp, tw);
   const int opts = cp.out_len * T;
   for (int i = threadIdx.x; i < opts; i += blockDim.x) {
     const int pp = tdiv.div(i);
-    const int col = c0 + i - pp * T;
-    if (col < cp.ncols) dst[pp * cp.pstride_out + col] = tile[i];
+    const int l = i - pp * T;
+    const int col = c0 + l;
+    if (col < cp.ncols) dst[pp * cp.pstride_out + col] = tile[pp * lp + l];
   }
 }
 
 // Pass 3 over the (ky, kx tile) pencil: for each component load z < Bz, FFT,
 // keep the spectrum in registers; multiply by D(kx, ky, kz); inverse FFT each
 // component and store z < Bz back in place.
+//
+// Work-groups are mapped XCD-aware: the four pencils that read the same octant
+// of the tensor, (kx tile, ky), its x mirror, its y mirror and both, are
+// consecutive work items, and consecutive work items go to one XCD (work-group
+// b runs on XCD b mod 8), so three of the four read the tensor from that XCD's
+// L2. Mirror duplicates (ky = 0, Gy/2, self-mirrored tiles) return at once.
 template <typename R, int PPT>
 __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
                                                MvDims d, FftPlan pz, FastDiv tdiv,
-                                               const typename CxT<R>::T* __restrict__ tw, const int* done) {
+                                               const typename CxT<R>::T* __restrict__ tw, int nwork, int nper,
+                                               const int* done) {
   typedef typename CxT<R>::T C;
   if (stop_requested(done)) return;
+  const int T = (int)tdiv.d;
+  const int ntx = (d.Gx + T - 1) / T;
+  const int ftn = (ntx + 1) / 2;
+  const int hx = d.Gx / 2, hy = d.Gy / 2, hz = d.Gz / 2;
+  const int w = (blockIdx.x & 7) * nper + (blockIdx.x >> 3);
+  if (w >= nwork) return;
+  const int sub = w & 3, grp = w >> 2;
+  const int ft = grp % ftn, fyp = grp / ftn;
+  int ky = fyp, tile_x = ft;
+  if (sub & 2) {
+    if (fyp == 0 || fyp == hy) return;
+    ky = d.Gy - fyp;
+  }
+  if (sub & 1) {
+    tile_x = ntx - 1 - ft;
+    if (tile_x == ft) return;
+  }
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
-  const int T = (int)tdiv.d;
-  const int c0 = blockIdx.x * T;
-  const int ky = blockIdx.y;
+  const int c0 = tile_x * T;
+  const int lp = lds_stride(T);
   const int N = d.Gz;
   const int pts = N * T;
   const long long cstride = (long long)d.Bz * d.Gy * d.Gx;
   const long long zstride = (long long)d.Gy * d.Gx;
+  const long long oct = (long long)(hx + 1) * (hy + 1) * (hz + 1);
   C* base = a2 + (long long)ky * d.Gx;
   C reg[3][PPT];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     for (int i = threadIdx.x; i < pts; i += blockDim.x) {
       const int z = tdiv.div(i);
-      const int col = c0 + i - z * T;
+      const int l = i - z * T;
+      const int col = c0 + l;
       C v = czero<C>();
       if (z < d.Bz && col < d.Gx) v = base[c * cstride + z * zstride + col];
-      tile[i] = v;
+      tile[z * lp + l] = v;
     }
     __syncthreads();
-    fft_tile<PPT, false>(tile, pz, tdiv, tw);
+    fft_tile<PPT, false>(tile, pz, tdiv, lp, tw);
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int i = threadIdx.x + k * blockDim.x;
-      if (i < pts) reg[c][k] = tile[i];
+      if (i < pts) {
+        const int z = tdiv.div(i);
+        reg[c][k] = tile[z * lp + i - z * T];
+      }
     }
     __syncthreads();
   }
-  const int hx = d.Gx / 2, hy = d.Gy / 2, hz = d.Gz / 2;
   const int fy = ky <= hy ? ky : d.Gy - ky;
   const R sy = ky <= hy ? (R)1 : (R)-1;
 #pragma unroll
@@ -241,9 +275,9 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
         const int fz = kz <= hz ? kz : d.Gz - kz;
         const R sx = kx <= hx ? (R)1 : (R)-1;
         const R sz = kz <= hz ? (R)1 : (R)-1;
-        const C* D = dhat + (((long long)fz * (hy + 1) + fy) * (hx + 1) + fx) * 6;
-        const C d0 = D[0], d3 = D[3], d5 = D[5];
-        const C d1 = cscale(D[1], sx * sy), d2 = cscale(D[2], sx * sz), d4 = cscale(D[4], sy * sz);
+        const C* D = dhat + ((long long)fz * (hy + 1) + fy) * (hx + 1) + fx;
+        const C d0 = D[0], d3 = D[3 * oct], d5 = D[5 * oct];
+        const C d1 = cscale(D[oct], sx * sy), d2 = cscale(D[2 * oct], sx * sz), d4 = cscale(D[4 * oct], sy * sz);
         const C x0 = reg[0][k], x1 = reg[1][k], x2 = reg[2][k];
         reg[0][k] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
         reg[1][k] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
@@ -256,17 +290,137 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int i = threadIdx.x + k * blockDim.x;
-      if (i < pts) tile[i] = reg[c][k];
+      if (i < pts) {
+        const int z = tdiv.div(i);
+        tile[z * lp + i - z * T] = reg[c][k];
+      }
     }
     __syncthreads();
-    fft_tile<PPT, true>(tile, pz, tdiv, tw);
+    fft_tile<PPT, true>(tile, pz, tdiv, lp, tw);
     const int opts = d.Bz * T;
     for (int i = threadIdx.x; i < opts; i += blockDim.x) {
       const int z = tdiv.div(i);
-      const int col = c0 + i - z * T;
-      if (col < d.Gx) base[c * cstride + z * zstride + col] = tile[i];
+      const int l = i - z * T;
+      const int col = c0 + l;
+      if (col < d.Gx) base[c * cstride + z * zstride + col] = tile[z * lp + l];
     }
     __syncthreads();
+  }
+}
+
+
+// ------------------------------------------------------------------------------------
+// Compile-time planned variants of the column passes (fft_static.hpp). Thread
+// (l, t) = (tid mod T, tid / T) owns column c0 + l and the plan's points t + ...
+// ------------------------------------------------------------------------------------
+
+// Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
+template <typename R, int N, int T>
+__global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
+                                                MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
+                                                int nper, const int* done) {
+  typedef typename CxT<R>::T C;
+  typedef StaticPlan<N, sizeof(R) == 4> P;
+  constexpr int PPT = P::PPT;
+  typedef typename P::Fwd F;
+  typedef typename Reverse<F>::type Inv;
+  if (stop_requested(done)) return;
+  const int ntx = (d.Gx + T - 1) / T;
+  const int ftn = (ntx + 1) / 2;
+  const int hx = d.Gx / 2, hy = d.Gy / 2, hz = N / 2;
+  const int w = (blockIdx.x & 7) * nper + (blockIdx.x >> 3);
+  if (w >= nwork) return;
+  const int sub = w & 3, grp = w >> 2;
+  const int ft = grp % ftn, fyp = grp / ftn;
+  int ky = fyp, tile_x = ft;
+  if (sub & 2) {
+    if (fyp == 0 || fyp == hy) return;
+    ky = d.Gy - fyp;
+  }
+  if (sub & 1) {
+    tile_x = ntx - 1 - ft;
+    if (tile_x == ft) return;
+  }
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  constexpr int lp = (T & 1) ? T : T + 1;
+  const int l = threadIdx.x % T, t = threadIdx.x / T;
+  const int kx = tile_x * T + l;
+  const bool colok = kx < d.Gx;
+  const long long cstride = (long long)d.Bz * d.Gy * d.Gx;
+  const long long zstride = (long long)d.Gy * d.Gx;
+  C* base = a2 + (long long)ky * d.Gx + kx;
+  C reg[3][PPT];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const int z = first_point<N, PPT, F>(t, e);
+      reg[c][e] = (colok && z < d.Bz) ? base[c * cstride + z * zstride] : czero<C>();
+    }
+#pragma unroll
+  for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F>::run(reg[c], t, tile, lp, l, tw);
+  const long long oct = (long long)(hx + 1) * (hy + 1) * (hz + 1);
+  const int fy = ky <= hy ? ky : d.Gy - ky;
+  const R sy = ky <= hy ? (R)1 : (R)-1;
+  const int fx = kx <= hx ? kx : d.Gx - kx;
+  const R sx = kx <= hx ? (R)1 : (R)-1;
+  if (colok) {
+    const C* Dxy = dhat + (long long)fy * (hx + 1) + fx;
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const int kz = first_point<N, PPT, Inv>(t, e);
+      const int fz = kz <= hz ? kz : N - kz;
+      const R sz = kz <= hz ? (R)1 : (R)-1;
+      const C* D = Dxy + (long long)fz * (hy + 1) * (hx + 1);
+      const C d0 = D[0], d3 = D[3 * oct], d5 = D[5 * oct];
+      const C d1 = cscale(D[oct], sx * sy), d2 = cscale(D[2 * oct], sx * sz), d4 = cscale(D[4 * oct], sy * sz);
+      const C x0 = reg[0][e], x1 = reg[1][e], x2 = reg[2][e];
+      reg[0][e] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
+      reg[1][e] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
+      reg[2][e] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    RunStages<N, PPT, true, 1, C, Inv>::run(reg[c], t, tile, lp, l, tw);
+#pragma unroll
+    for (int e = 0; e < PPT; e++) {
+      const int z = first_point<N, PPT, F>(t, e);
+      if (colok && z < d.Bz) base[c * cstride + z * zstride] = reg[c][e];
+    }
+  }
+}
+
+// Column FFT with a static plan (pass 2 / pass 4 and the tensor generation).
+template <typename R, int N, int T, bool INV>
+__global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
+                                                  const typename CxT<R>::T* __restrict__ tw, const int* done) {
+  typedef typename CxT<R>::T C;
+  typedef StaticPlan<N, sizeof(R) == 4> P;
+  constexpr int PPT = P::PPT;
+  typedef typename P::Fwd F;
+  typedef typename Reverse<F>::type Out;
+  if (stop_requested(done)) return;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  constexpr int lp = (T & 1) ? T : T + 1;
+  const int l = threadIdx.x % T, t = threadIdx.x / T;
+  const int col = blockIdx.x * T + l;
+  const bool colok = col < cp.ncols;
+  const C* src = in + (long long)blockIdx.y * cp.bstride_in + col;
+  C* dst = out + (long long)blockIdx.y * cp.bstride_out + col;
+  C v[PPT];
+#pragma unroll
+  for (int e = 0; e < PPT; e++) {
+    const int p = first_point<N, PPT, F>(t, e);
+    v[e] = (colok && p < cp.in_len) ? src[p * cp.pstride_in] : czero<C>();
+  }
+  RunStages<N, PPT, INV, 1, C, F>::run(v, t, tile, lp, l, tw);
+#pragma unroll
+  for (int e = 0; e < PPT; e++) {
+    const int p = first_point<N, PPT, Out>(t, e);
+    if (colok && p < cp.out_len) dst[p * cp.pstride_out] = v[e];
   }
 }
 
@@ -278,6 +432,7 @@ __global__ __launch_bounds__(256) void k_rowfft(typename CxT<R>::T* data, long l
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = (int)ldiv.d;
+  const int lp = lds_stride(L);
   const int N = p.n;
   const int pts = N * L;
   const long long l0 = (long long)blockIdx.x * L;
@@ -286,14 +441,14 @@ __global__ __launch_bounds__(256) void k_rowfft(typename CxT<R>::T* data, long l
     const int pp = i - l * N;
     C v = czero<C>();
     if (l0 + l < nlines) v = data[(l0 + l) * N + pp];
-    tile[pp * L + l] = v;
+    tile[pp * lp + l] = v;
   }
   __syncthreads();
-  fft_tile<PPT, INV>(tile, p, ldiv, tw);
+  fft_tile<PPT, INV>(tile, p, ldiv, lp, tw);
   for (int i = threadIdx.x; i < pts; i += blockDim.x) {
     const int l = ndiv.div(i);
     const int pp = i - l * N;
-    if (l0 + l < nlines) data[(l0 + l) * N + pp] = tile[pp * L + l];
+    if (l0 + l < nlines) data[(l0 + l) * N + pp] = tile[pp * lp + l];
   }
 }
 
@@ -346,7 +501,7 @@ __global__ void k_dhat_extract(const double2* __restrict__ w, typename CxT<R>::T
     const int fy = (int)((t / hx) % hy);
     const int fz = (int)(t / ((long long)hx * hy));
     const double2 v = w[((long long)fz * Gy + fy) * Gx + fx];
-    dhat[t * 6 + comp] = cmake<C>(v.x * scale, v.y * scale);
+    dhat[comp * total + t] = cmake<C>(v.x * scale, v.y * scale);
   }
 }
 
@@ -425,7 +580,7 @@ dda_status launch_col(const void* in, void* out, const ColParams& cp, const FftP
                       int nbatch, hipStream_t st, const int* done) {
   typedef typename CxT<R>::T C;
   const int pts = p.n * T;
-  const size_t lds = (size_t)pts * sizeof(C);
+  const size_t lds = (size_t)p.n * lds_stride(T) * sizeof(C);
   dim3 grid((cp.ncols + T - 1) / T, nbatch);
   FastDiv tdiv = make_fastdiv(T);
   if (pts <= 8 * 256) {
@@ -458,7 +613,7 @@ dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void
     set_error("row FFT too long");
     return DDA_NOT_SUPPORTED;
   }
-  const size_t lds = (size_t)pts * sizeof(C);
+  const size_t lds = (size_t)p.n * lds_stride(L) * sizeof(C);
   dim3 grid((unsigned)((nlines + L - 1) / L));
   FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(p.n);
   if (ppt == 8) {
@@ -472,6 +627,79 @@ dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void
   }
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
+}
+
+
+// Columns per work-group for a static plan: 128-byte row segments, at most 256
+// threads.
+template <typename R, int N>
+constexpr int static_cols() {
+  constexpr int pref = sizeof(R) == 8 ? 8 : 16;
+  constexpr int tpc = N / StaticPlan<N, sizeof(R) == 4>::PPT;
+  return pref * tpc <= 256 ? pref : 256 / tpc;
+}
+
+template <typename R, int N, bool INV>
+dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
+                             const int* done) {
+  typedef typename CxT<R>::T C;
+  constexpr int T = static_cols<R, N>();
+  constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
+  constexpr int lp = (T & 1) ? T : T + 1;
+  const size_t lds = (size_t)N * lp * sizeof(C);
+  auto k = k_colfft_s<R, N, T, INV>;
+  DDA_TRY(set_lds(k, lds));
+  hipLaunchKernelGGL(k, dim3((cp.ncols + T - 1) / T, nbatch), dim3(threads), lds, st, (const C*)in, (C*)out, cp,
+                     (const C*)tw, done);
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
+template <typename R, int N>
+dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, const int* done) {
+  typedef typename CxT<R>::T C;
+  constexpr int T = static_cols<R, N>();
+  constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
+  constexpr int lp = (T & 1) ? T : T + 1;
+  const MvDims& d = op->dims;
+  const size_t lds = (size_t)N * lp * sizeof(C);
+  const int ntx = (d.Gx + T - 1) / T;
+  const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4;
+  const int nper = (nwork + 7) / 8;
+  auto k = k_pass3s<R, N, T>;
+  DDA_TRY(set_lds(k, lds));
+  hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)op->a2, (const C*)op->dhat, d,
+                     (const C*)op->tw[2], nwork, nper, done);
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
+// Lengths with a static plan; others use the run-time planned kernels.
+#define DDA_STATIC_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(512) X(1024)
+
+inline bool has_static_plan(int n) {
+  switch (n) {
+#define X(N) case N:
+    DDA_STATIC_SIZES(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <typename R, bool INV>
+dda_status launch_col_any(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
+                          int nbatch, hipStream_t st, const int* done) {
+  switch (p.n) {
+#define X(N) \
+  case N:    \
+    return launch_col_static<R, N, INV>(in, out, cp, tw, nbatch, st, done);
+    DDA_STATIC_SIZES(X)
+#undef X
+    default:
+      return launch_col<R, INV>(in, out, cp, p, T, tw, nbatch, st, done);
+  }
 }
 
 }  // namespace
@@ -525,7 +753,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   const int L = 3 * lr;
   const int pts_x = d.Gx * L;
   const int ppt_x = pts_x <= 8 * 256 ? 8 : 16;
-  const size_t lds_x = (size_t)pts_x * sizeof(C);
+  const size_t lds_x = (size_t)d.Gx * lds_stride(L) * sizeof(C);
   const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
   {
     PassTimer pt(op, st, 0);
@@ -554,26 +782,41 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.By * d.Gx;
     cp.bstride_out = (long long)d.Gy * d.Gx;
-    DDA_TRY((launch_col<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
-  {
+  if (op->static_z) {
+    PassTimer pt(op, st, 2);
+    switch (d.Gz) {
+#define X(N)                                               \
+  case N:                                                  \
+    DDA_TRY((launch_pass3_static<R, N>(op, st, done))); \
+    break;
+      DDA_STATIC_SIZES(X)
+#undef X
+      default:
+        break;
+    }
+  } else {
     PassTimer pt(op, st, 2);
     const int T = op->col_t;
     const int pts = d.Gz * T;
-    const size_t lds = (size_t)pts * sizeof(C);
-    dim3 grid((d.Gx + T - 1) / T, d.Gy);
+    const size_t lds = (size_t)d.Gz * lds_stride(T) * sizeof(C);
+    const int ntx = (d.Gx + T - 1) / T;
+    const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4;
+    const int nper = (nwork + 7) / 8;
+    dim3 grid(8 * nper);
     FastDiv tdiv = make_fastdiv(T);
     if (pts <= 8 * 256) {
       auto k = k_pass3<R, 8>;
       DDA_TRY(set_lds(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], done);
+                         (const C*)op->tw[2], nwork, nper, done);
     } else if (pts <= 16 * 256) {
       auto k = k_pass3<R, 16>;
       DDA_TRY(set_lds(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], done);
+                         (const C*)op->tw[2], nwork, nper, done);
     } else {
       set_error("Z pass tile too large");
       return DDA_NOT_SUPPORTED;
@@ -591,7 +834,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.Gy * d.Gx;
     cp.bstride_out = (long long)d.By * d.Gx;
-    DDA_TRY((launch_col<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
   }
   // pass 5
   {
@@ -641,14 +884,14 @@ static dda_status generate_dhat(dda_op_s* op) {
     cy.in_len = cy.out_len = d.Gy;
     cy.pstride_in = cy.pstride_out = d.Gx;
     cy.bstride_in = cy.bstride_out = (long long)d.Gx * d.Gy;
-    rc = launch_col<double, false>(w, w, cy, op->plan[1], col_tile(d.Gy, 8), tw64[1], d.Gz, st, nullptr);
+    rc = launch_col_any<double, false>(w, w, cy, op->plan[1], col_tile(d.Gy, 8), tw64[1], d.Gz, st, nullptr);
     if (rc != DDA_OK) break;
     ColParams cz;
     cz.ncols = d.Gx * d.Gy;
     cz.in_len = cz.out_len = d.Gz;
     cz.pstride_in = cz.pstride_out = (long long)d.Gx * d.Gy;
     cz.bstride_in = cz.bstride_out = 0;
-    rc = launch_col<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, nullptr);
+    rc = launch_col_any<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, nullptr);
     if (rc != DDA_OK) break;
     if (op->prec == 64)
       hipLaunchKernelGGL(k_dhat_extract<double>, dim3(1024), dim3(256), 0, st, w, (double2*)op->dhat, comp, d.Gx, d.Gy,
@@ -786,6 +1029,7 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
     }
   }
   op->n_nonempty = (int)row_list.size();
+  op->static_z = has_static_plan(d.Gz) && !std::getenv("DDA_GENERIC_FFT");
 
   // Launch shapes: X passes hold ~2048 points per work-group; column passes
   // take 128-byte row segments (8 complex doubles, 16 complex floats).
@@ -850,7 +1094,7 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
         for (int fx = 0; fx < hx; fx++)
           for (int c = 0; c < 6; c++) {
             const size_t src = (((size_t)fy + (size_t)d.Gy * (fz + (size_t)d.Gz * fx)) * 6 + c) * 2;
-            const size_t dst = ((((size_t)fz * hy + fy) * hx + fx) * 6 + c);
+            const size_t dst = (size_t)c * hx * hy * hz + (((size_t)fz * hy + fy) * hx + fx);
             const double re = ds.dmatrix[src] * d3, im = ds.dmatrix[src + 1] * d3;
             if (precision == 64)
               reinterpret_cast<double2*>(h.data())[dst] = make_double2(re, im);
@@ -994,7 +1238,7 @@ extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
         const double sx = i <= d.Gx / 2 ? 1 : -1, sy = j <= d.Gy / 2 ? 1 : -1, sz = k <= d.Gz / 2 ? 1 : -1;
         const double sgn[6] = {1, sx * sy, sx * sz, 1, sy * sz, 1};
         for (int c = 0; c < 6; c++) {
-          const size_t src = (((size_t)fz * hy + fy) * hx + fx) * 6 + c;
+          const size_t src = (size_t)c * hx * hy * hz + (((size_t)fz * hy + fy) * hx + fx);
           double re, im;
           if (op->prec == 64) {
             double2 v = reinterpret_cast<double2*>(h.data())[src];

@@ -52,6 +52,7 @@ def lib():
         L.orc_einc_plane.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, _dp]
         L.orc_dmatrix.argtypes = [_u32p, _u32p, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_dmatrix64.argtypes = [_u32p, _u32p, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_green.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_matvec.argtypes = [ctypes.POINTER(_Prob), _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_matvec_direct.argtypes = [ctypes.POINTER(_Prob), ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
@@ -145,12 +146,14 @@ def einc_plane(pos, box, vec_stride, grid_unit, wave_num, prop, pol):
     return out
 
 
-def dmatrix(box, grid, grid_unit, wave_num):
-    """DMatrixCpu.cpp:133-205; returns complex array [Gx, Gz, Gy, 6] (reference memory order)."""
+def dmatrix(box, grid, grid_unit, wave_num, precision=80):
+    """DMatrixCpu.cpp:133-205; returns complex array [Gx, Gz, Gy, 6] (reference memory order).
+    precision 80 computes in long double, 64 in double."""
     box = np.ascontiguousarray(box, np.uint32)
     grid = np.ascontiguousarray(grid, np.uint32)
     out = np.zeros((int(grid[0]), int(grid[2]), int(grid[1]), 6), np.complex128)
-    st = lib().orc_dmatrix(_p(box, _u32p), _p(grid, _u32p), grid_unit, wave_num, _p(out.view(np.float64), _dp))
+    f = lib().orc_dmatrix if precision == 80 else lib().orc_dmatrix64
+    st = f(_p(box, _u32p), _p(grid, _u32p), grid_unit, wave_num, _p(out.view(np.float64), _dp))
     if st != ST_OK:
         raise ValueError("grid too small for box")
     return out
@@ -165,7 +168,8 @@ def green(r, wave_num, mu, nu):
 class Problem:
     """Geometry + grid + couple constants + D matrix, as the CPU path holds them."""
 
-    def __init__(self, pos, mat, box, grid, cc_sqrt, grid_unit, wave_num, dmat=None, vec_stride=None):
+    def __init__(self, pos, mat, box, grid, cc_sqrt, grid_unit, wave_num, dmat=None, vec_stride=None,
+                 dmat_precision=80):
         self.pos = np.ascontiguousarray(pos, np.uint32)
         self.nv = self.pos.shape[0]
         self.mat = np.ascontiguousarray(mat if mat is not None else np.zeros(self.nv), np.uint8)
@@ -175,7 +179,8 @@ class Problem:
         self.grid_unit = float(grid_unit)
         self.wave_num = float(wave_num)
         self.vec_stride = int(vec_stride) if vec_stride else ((self.nv + 15) // 16) * 16
-        self.dmat = _cd(dmat) if dmat is not None else dmatrix(self.box, self.grid, self.grid_unit, self.wave_num)
+        self.dmat = _cd(dmat) if dmat is not None else dmatrix(self.box, self.grid, self.grid_unit, self.wave_num,
+                                                                 dmat_precision)
         self._s = _Prob()
         self._s.box[:] = [int(v) for v in self.box]
         self._s.grid[:] = [int(v) for v in self.grid]

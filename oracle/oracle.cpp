@@ -668,7 +668,11 @@ void orc_einc_plane(uint32_t nv, uint32_t vec_stride, const uint32_t* pos, const
 // D matrix in the reference layout, DMatrixCpu.cpp:133-205:
 // out[((j + Gy*(k + Gz*i))*6 + comp)] = FFT3(G clipped to |offset| < box)/(-Gx*Gy*Gz).
 // Computed in long double, returned as double.
-int orc_dmatrix(const uint32_t* box, const uint32_t* grid, double gridUnit, double waveNum, double* out) {
+}  // extern "C"
+
+template <typename TT>
+static int dmatrix_impl(const uint32_t* box, const uint32_t* grid, double gridUnit, double waveNum, double* out) {
+  typedef TT ld;
   typedef std::complex<ld> C;
   const int Bx = box[0], By = box[1], Bz = box[2];
   const int Gx = grid[0], Gy = grid[1], Gz = grid[2];
@@ -712,6 +716,17 @@ int orc_dmatrix(const uint32_t* box, const uint32_t* grid, double gridUnit, doub
     }
   }
   return ST_OK;
+}
+
+extern "C" {
+
+int orc_dmatrix(const uint32_t* box, const uint32_t* grid, double gridUnit, double waveNum, double* out) {
+  return dmatrix_impl<long double>(box, grid, gridUnit, waveNum, out);
+}
+
+// The same in double (for large grids in tests).
+int orc_dmatrix64(const uint32_t* box, const uint32_t* grid, double gridUnit, double waveNum, double* out) {
+  return dmatrix_impl<double>(box, grid, gridUnit, waveNum, out);
 }
 
 // Green tensor component for one offset (metres), for closed-form checks.

@@ -50,9 +50,10 @@ def build(name, grid=(0, 0, 0), support_non_pot=True, prop=(0, 0, 1), pol_index=
     return g, params, cc, cs, prop_n, pol
 
 
-def oracle_problem(g, params, cs):
+def oracle_problem(g, params, cs, dmat_precision=80):
     import oracle as O
-    return O.Problem(g.pos, g.mat, g.box, params.grid, cs, g.grid_unit, params.wave_num)
+    return O.Problem(g.pos, g.mat, g.box, params.grid, cs, g.grid_unit, params.wave_num,
+                     dmat_precision=dmat_precision)
 
 
 def rel_err(a, b):

@@ -62,6 +62,25 @@ def test_matvec_grid_choices(ctx, grid, pot):
     assert cases.rel_err(y, y_ref) <= FP64_TOL
 
 
+# Grid lengths with compile-time planned column kernels, each on the Y (passes
+# 2, 4) and the Z axis (pass 3); the 10x8x6 ellipsoid box embeds exactly in all.
+STATIC_GRIDS = [(32, 64, 96), (32, 96, 64), (32, 128, 192), (32, 192, 128), (32, 256, 512), (32, 512, 256),
+                (32, 1024, 16), (32, 16, 1024)]
+
+
+@pytest.mark.parametrize("grid", STATIC_GRIDS)
+@pytest.mark.parametrize("precision", [64, 32])
+def test_matvec_static_plans(ctx, grid, precision):
+    print("grid", grid, "precision", precision, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("ellipsoid_5_4_3", grid=grid)
+    assert params.grid == grid
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params, seed=11)
+    y_ref = orc.matvec(x, precision=64)
+    y = D.MatVecGpu(_ops(ctx, g, params, cs, precision)).apply(x)
+    assert cases.rel_err(y, y_ref) <= (FP64_TOL if precision == 64 else FP32_TOL)
+
+
 def test_dmatrix_device_generation_matches_oracle(ctx):
     """FP64 tensor generated on the GPU vs DMatrixCpu::createDMatrix restated."""
     import oracle as O

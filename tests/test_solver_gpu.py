@@ -61,7 +61,8 @@ def test_qmr_c1_solve(ctx):
     orc = cases.oracle_problem(g, params, cs)
     st, p_ref, it_ref, _ = orc.solve("qmr", einc, 1e-8, max_iter=2000)
     assert st == 0
-    assert abs(s.info["iterations"] - it_ref) <= 2
+    # ~150 iterations to 1e-8: rounding shifts the count by a few percent at most
+    assert abs(s.info["iterations"] - it_ref) <= max(2, it_ref // 25), (s.info["iterations"], it_ref)
     assert abs(cext - orc.ext_cross_sum(p_ref, einc)) <= 1e-6 * abs(cext)
 
 
