@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed", help="solver modes to time; value = the last one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU slab "
@@ -104,33 +105,46 @@ def main():
     t0 = time.perf_counter()
     op = D.GpuMatVec(ctx, g.pos, g.mat, box, d, params.wave_num, precision=64, grid=grid)
     t_setup = time.perf_counter() - t0
-    solver = D.SOLVER_CLASSES[args.solver](op, max_iter=20000)
+    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
 
-    results = []
+    def run_mode(mode):
+        solver = D.SOLVER_CLASSES[args.solver](op, max_iter=20000, mixed=(mode == "mixed"))
 
-    def step():
-        out = []
-        for pol, cc, cs, einc in rhs:
-            solver.set_couple_constants(cs)
-            p = solver.get_pol_vec(einc, args.eps)
-            out.append((p, dict(solver.info), cc, einc, pol))
-        return out
+        def step():
+            out = []
+            for pol, cc, cs, einc in rhs:
+                solver.set_couple_constants(cs)
+                p = solver.get_pol_vec(einc, args.eps)
+                out.append((p, dict(solver.info), cc, einc, pol))
+            return out
 
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    op.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        results = step()
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms, cnt, skipped = op.profile_read()
-    op.profile(False)
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        op.profile(True)
+        op.profile_read()
+        t0 = time.perf_counter()
+        res = []
+        for _ in range(args.steps):
+            res = step()
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        prof = solver.profile_read()  # the operator the iteration runs on
+        op.profile_read()
+        op.profile(False)
+        solver.close()
+        return res, el, prof
+
+    runs = {}
+    for mode in modes:
+        runs[mode] = run_mode(mode)
+    final_mode = modes[-1]
+    results, elapsed, _ = runs[final_mode]
+    ms, cnt, skipped = runs[final_mode][2]
 
     nsolves = 2 * args.steps
     value = nsolves / elapsed
-    c = 16
+    c = 8 if final_mode == "mixed" else 16
     pb = pass_bytes(nv, S, box, grid, rows, c)
     avg_ms = [ms[i] / cnt[i] if cnt[i] else float("nan") for i in range(5)]
     dom = int(np.nanargmax(ms))
@@ -139,14 +153,40 @@ def main():
     bmv = b_mv(nv, box, grid, c)
 
     # observables of the last step, checked against Mie theory (volume-equivalent sphere)
-    obs = []
     a_eq = (3 * nv / (4 * math.pi)) ** (1 / 3) * d
-    for p, info, cc, einc, pol in results:
-        cext = P.ext_cross(params, p, einc)
-        cabs = P.abs_cross(params, p, cc)
-        obs.append({"iterations": int(info["iterations"]), "true_rel_residual": info["true_rel_residual"],
-                    "seconds": info["seconds"], "cext": cext, "cabs": cabs,
-                    "cext_forward": P.ext_cross_forward(params, p, prop, pol)})
+
+    def observe(res):
+        out = []
+        for p, info, cc, einc, pol in res:
+            out.append({"iterations": int(info["iterations"]), "refinements": int(info["refinements"]),
+                        "true_rel_residual": info["true_rel_residual"], "seconds": info["seconds"],
+                        "cext": P.ext_cross(params, p, einc), "cabs": P.abs_cross(params, p, cc),
+                        "cext_forward": P.ext_cross_forward(params, p, prop, pol)})
+        return out
+
+    obs = observe(results)
+    modes_out = {}
+    for mode, (res, el, prof) in runs.items():
+        o = observe(res)
+        pms, pcnt, _ = prof
+        cm = 8 if mode == "mixed" else 16
+        pbm = pass_bytes(nv, S, box, grid, rows, cm)
+        pav = [pms[i] / pcnt[i] if pcnt[i] else float("nan") for i in range(5)]
+        modes_out[mode] = {"solves_per_s": nsolves / el, "ms_per_step": 1e3 * el / args.steps,
+                           "matvec_ms": float(np.nansum(pav)),
+                           "matvec_gbs_vs_bmv": b_mv(nv, box, grid, cm) / (float(np.nansum(pav)) * 1e-3) / 1e9,
+                           "pass_avg_us": [1e3 * v for v in pav],
+                           "pass_gbs": [pbm[i] / (pav[i] * 1e-3) / 1e9 for i in range(5)],
+                           "iterations": [x["iterations"] for x in o],
+                           "refinements": [x["refinements"] for x in o],
+                           "true_rel_residual": [x["true_rel_residual"] for x in o],
+                           "cext": [x["cext"] for x in o], "cabs": [x["cabs"] for x in o]}
+    if "fp64" in modes_out and "mixed" in modes_out:
+        f, m_ = modes_out["fp64"], modes_out["mixed"]
+        modes_out["mixed_vs_fp64"] = {
+            "speedup": m_["solves_per_s"] / f["solves_per_s"],
+            "cext_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cext"], f["cext"])),
+            "cabs_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cabs"], f["cabs"]))}
     mie = None
     try:
         import oracle as O
@@ -189,21 +229,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if final_mode == "mixed" else "f64",
         "data": "synthetic (generated sphere geometry, plane-wave incident field)",
         "config": {"workload": "sphere %d dipoles across (nv=%d), m=1.5+0.1i, |m|kd=0.5, grid %s, prop z, "
-                               "2 polarisations, %s to resid %g, FP64" % (args.diameter, nv, "x".join(map(str, grid)),
-                                                                        args.solver, args.eps),
-                   "nv": nv, "grid": list(grid), "solver": args.solver, "mode": "fp64", "eps": args.eps},
+                               "2 polarisations, %s to resid %g, %s" % (args.diameter, nv, "x".join(map(str, grid)),
+                                                                      args.solver, args.eps, final_mode),
+                   "nv": nv, "grid": list(grid), "solver": args.solver, "mode": final_mode, "eps": args.eps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": PASS_NAMES[dom],
                      "bytes_per_launch": pb[dom], "avg_launch_us": 1e3 * avg_ms[dom],
-                     "launches": int(cnt[dom]), "noop_launches_excluded": skipped},
+                     "launches": int(cnt[dom]), "noop_launches_excluded": skipped,
+                     "measured_in": "%s timed region (%s kernels)" % (final_mode,
+                                                                       "FP32" if c == 8 else "FP64")},
         "matvec": {"ms": mv_ms, "bmv_bytes": bmv, "gbs_vs_bmv": bmv / (mv_ms * 1e-3) / 1e9,
                    "frac_vs_bmv": bmv / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "pass_avg_us": [1e3 * v for v in avg_ms], "pass_bytes": pb,
                    "pass_gbs": [pb[i] / (avg_ms[i] * 1e-3) / 1e9 for i in range(5)]},
         "solves": obs,
+        "modes": modes_out,
         "mie": mie,
         "setup_s": t_setup,
         "device": info,

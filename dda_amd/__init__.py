@@ -26,6 +26,7 @@ OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, MAX_ITER, STAGNATION, NOT_
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "BREAKDOWN",
                 5: "MAX_ITER", 6: "STAGNATION", 7: "NOT_SUPPORTED"}
 QMR_CS, BICGSTAB, CGNR, BICG_CS = 0, 1, 2, 3
+MODE_FP64, MODE_MIXED = 0, 1
 SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_CS}
 
 # Every symbol include/dda_hip.h declares.
@@ -34,7 +35,8 @@ EXPORTS = [
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_fft_fit",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_apply",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
-    "dda_solver_create", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
+    "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
+    "dda_solver_set_inner_floor", "dda_solver_profile_read",
 ]
 
 
@@ -99,6 +101,9 @@ def lib():
         L.dda_op_profile.argtypes = [vp, i32]
         L.dda_op_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_create.argtypes = [vp, i32, u64, u64, pp]
+        L.dda_solver_create_ex.argtypes = [vp, i32, u64, u64, i32, pp]
+        L.dda_solver_set_inner_floor.argtypes = [vp, dbl]
+        L.dda_solver_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_destroy.argtypes = [vp]
         L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_history.argtypes = [vp, dp, u64, ctypes.POINTER(u64)]
@@ -329,15 +334,22 @@ class MatVecGpu:
 
 
 class IterativeSolver:
-    """IterativeSolverBase<T> over a GpuMatVec (FP64): getPolVec(einc, eps, start)."""
+    """IterativeSolverBase<T> over an FP64 GpuMatVec: getPolVec(einc, eps, start).
+
+    mixed=True runs the Krylov iteration in FP32 inside FP64 iterative
+    refinement (no counterpart in the reference); `inner_floor` is the
+    tolerance floor of the FP32 inner solves."""
 
     kind = QMR_CS
 
-    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0):
+    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0, mixed=False, inner_floor=None):
         self.op = gpu_matvec
+        self.mixed = bool(mixed)
         h = ctypes.c_void_p()
-        _check(lib().dda_solver_create(gpu_matvec.handle, self.kind, int(max_iter), int(max_res_increase),
-                                       ctypes.byref(h)), "dda_solver_create")
+        _check(lib().dda_solver_create_ex(gpu_matvec.handle, self.kind, int(max_iter), int(max_res_increase),
+                                          MODE_MIXED if mixed else MODE_FP64, ctypes.byref(h)), "dda_solver_create")
+        if inner_floor is not None:
+            _check(lib().dda_solver_set_inner_floor(h, float(inner_floor)))
         self.handle = h
         self.info = None
         gpu_matvec.ctx._adopt(self)
@@ -359,6 +371,14 @@ class IterativeSolver:
         return out
 
     getPolVec = get_pol_vec
+
+    def profile_read(self):
+        """Kernel times of the operator the iteration runs on (FP32 twin when mixed)."""
+        ms = np.zeros(5)
+        cnt = (ctypes.c_uint64 * 5)()
+        sk = ctypes.c_uint64()
+        _check(lib().dda_solver_profile_read(self.handle, _dp(ms), cnt, ctypes.byref(sk)))
+        return ms, np.array(list(cnt), np.int64), int(sk.value)
 
     def history(self):
         n = int(self.info["iterations"]) + 1 if self.info else 1

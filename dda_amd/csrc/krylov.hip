@@ -387,6 +387,10 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st, const d
   switch (step) {
     case S_INIT_TEMP:
       st->temp = q0;
+      if (!(q0 > 0)) {  // a zero right-hand side (residScale would be infinite)
+        finish(st, DDA_INVALID_ARG);
+        break;
+      }
       st->residScale = 1.0 / q0;
       st->epsB = eps * eps / st->residScale;
       st->one = dre(1.0);
@@ -657,256 +661,335 @@ __global__ __launch_bounds__(kRedThreads) void k_xr_update(typename CxT<R>::T* x
   write_partials(part, 1, sum, 0, 0, 0);
 }
 
+// Mixed precision helpers: r32 = (float)(r64 * scale); x64 += (double)d32 * scale;
+// r64 = b64 - t64 with partial |r|^2.
+__global__ void k_to_f32(float2* out, const double2* __restrict__ in, long long n, double scale) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)(in[i].x * scale), (float)(in[i].y * scale));
+}
+__global__ void k_add_f32(double2* x, const float2* __restrict__ d, long long n, double scale) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i].x += (double)d[i].x * scale;
+    x[i].y += (double)d[i].y * scale;
+  }
+}
+__global__ __launch_bounds__(kRedThreads) void k_resid64(double2* r, const double2* __restrict__ b,
+                                                         const double2* __restrict__ t, long long n, double* part) {
+  double s = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = make_double2(b[i].x - t[i].x, b[i].y - t[i].y);
+    r[i] = v;
+    s += v.x * v.x + v.y * v.y;
+  }
+  write_partials(part, 1, s, 0, 0, 0);
+}
+
+// One Krylov solve A x = b on the device for an operator of precision R.
+// v[0] = b (set by the caller), v[2] = x (x0 when have_x0), v[1] = r, v[3] = Av,
+// v[4..7] solver work vectors.
+template <typename R>
+struct Core {
+  typedef typename CxT<R>::T C;
+  dda_op_s* op = nullptr;
+  int kind = 0;
+  hipStream_t st = nullptr;
+  long long n = 0;
+  SolverState* dst = nullptr;       // device
+  SolverState* host2 = nullptr;     // pinned, two slots
+  double* part = nullptr;
+  double* hist = nullptr;
+  uint64_t hist_cap = 0;
+  uint64_t max_iter = 0, max_res_increase = 0;
+  C* v[8] = {};
+
+  dim3 g() const { return dim3(kRedBlocks); }
+  dim3 b() const { return dim3(kRedThreads); }
+  dda_status matvec(const void* x, void* y, int conj, const int* flag) {
+    return launch_matvec<R>(op, x, y, conj, st, flag);
+  }
+  void scalar(int step, double eps = 0) {
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kRedThreads), 0, st, dst, part, step, kind, eps, hist);
+  }
+  void reduce(int opk, const C* a, const C* bb, const int* flag) {
+    if (opk == RED_NORM)
+      hipLaunchKernelGGL((k_reduce<R, RED_NORM>), g(), b(), 0, st, a, bb, n, part, flag);
+    else if (opk == RED_DOTU)
+      hipLaunchKernelGGL((k_reduce<R, RED_DOTU>), g(), b(), 0, st, a, bb, n, part, flag);
+    else
+      hipLaunchKernelGGL((k_reduce<R, RED_DOTC>), g(), b(), 0, st, a, bb, n, part, flag);
+  }
+  void lincomb(C* out, const C* v1, const double2* c1, const C* v2, const double2* c2, const C* v3, bool withpart,
+               const int* flag) {
+    hipLaunchKernelGGL(k_lincomb<R>, g(), b(), 0, st, out, v1, c1, v2, c2, v3, n, withpart ? part : nullptr, flag);
+  }
+
+  dda_status alloc(dda_op_s* o, int k, uint64_t mi, uint64_t mri) {
+    op = o;
+    kind = k;
+    st = o->ctx->stream;
+    n = 3ll * o->dims.S;
+    max_iter = mi;
+    max_res_increase = mri;
+    hist_cap = std::min<uint64_t>(mi + 1, 1u << 20);
+    DDA_TRY_HIP(hipMalloc(&dst, sizeof(SolverState)));
+    DDA_TRY_HIP(hipHostMalloc((void**)&host2, 2 * sizeof(SolverState), hipHostMallocDefault));
+    DDA_TRY_HIP(hipMalloc(&part, 4 * kRedBlocks * sizeof(double)));
+    DDA_TRY_HIP(hipMalloc(&hist, hist_cap * sizeof(double)));
+    for (int i = 0; i < 8; i++) DDA_TRY_HIP(hipMalloc(&v[i], n * sizeof(C)));
+    return DDA_OK;
+  }
+  void release() {
+    if (dst) (void)hipFree(dst);
+    if (host2) (void)hipHostFree(host2);
+    if (part) (void)hipFree(part);
+    if (hist) (void)hipFree(hist);
+    for (C* p : v)
+      if (p) (void)hipFree(p);
+    dst = nullptr;
+    host2 = nullptr;
+    part = nullptr;
+    hist = nullptr;
+    for (auto& p : v) p = nullptr;
+  }
+
+  // Sum of partial quantity 0 on the host (synchronises).
+  dda_status read_sum(double* out) {
+    std::vector<double> h(kRedBlocks);
+    DDA_TRY_HIP(hipMemcpyAsync(h.data(), part, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipStreamSynchronize(st));
+    double t = 0;
+    for (double x : h) t += x;
+    *out = t;
+    return DDA_OK;
+  }
+
+  // initGeneral + getPolVec loop (GpuIterativeSolver.cpp:78-134,
+  // IterativeSolverBase.cpp:40-103); leaves x in v[2] and the final state in *fin.
+  dda_status solve(double eps, bool have_x0, SolverState* fin) {
+    const int* done = &dst->done;
+    const int* skip = &dst->skip;
+    C* bv = v[0];
+    C* r = v[1];
+    C* x = v[2];
+    C* av = v[3];
+    SolverState h;
+    std::memset(&h, 0, sizeof h);
+    h.max_iter = max_iter;
+    h.max_res_increase = max_res_increase;
+    h.hist_cap = hist_cap;
+    DDA_TRY_HIP(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, st));
+    reduce(RED_NORM, bv, bv, skip);
+    scalar(S_INIT_TEMP, eps);
+    if (have_x0) {
+      DDA_TRY(matvec(x, av, 0, nullptr));
+      lincomb(r, av, &dst->mOne, nullptr, nullptr, bv, true, skip);  // r = b - A x0
+      scalar(S_INIT_START);
+    } else {
+      DDA_TRY(matvec(bv, av, 0, nullptr));
+      lincomb(r, av, &dst->mOne, nullptr, nullptr, bv, true, skip);  // r = b - A b
+      scalar(S_INIT_R);
+      hipLaunchKernelGGL(k_select_x0<R>, g(), b(), 0, st, x, r, bv, n, dst);
+    }
+    if (kind == DDA_SOLVER_QMR_CS) {
+      reduce(RED_DOTU, r, r, skip);
+      scalar(S_QMR_INIT);
+      lincomb(v[4], r, &dst->betaInv, nullptr, nullptr, nullptr, false, skip);  // v = r / beta
+    } else if (kind == DDA_SOLVER_BICGSTAB) {
+      hipLaunchKernelGGL(k_copy_flag<R>, g(), b(), 0, st, v[7], r, n, skip);  // rtilda = r
+    }
+    scalar(S_LOOP_CHECK);
+    DDA_TRY_HIP(hipGetLastError());
+
+    int iv = 4, ivt = 5, ipo = 6, ipn = 7;
+    uint64_t nr = 0;
+    auto enqueue_iteration = [&]() -> dda_status {
+      switch (kind) {
+        case DDA_SOLVER_QMR_CS: {
+          C* vv = v[iv];
+          C* vt = v[ivt];
+          DDA_TRY(matvec(vv, av, 0, done));
+          reduce(RED_DOTU, vv, av, done);
+          scalar(S_QMR_ALPHA);
+          hipLaunchKernelGGL(k_qmr_vt<R>, g(), b(), 0, st, vt, vv, av, nr == 0 ? 1 : 0, dst, n, part);
+          scalar(S_QMR_GIVENS);
+          const int mode = nr == 0 ? 0 : (nr == 1 ? 1 : 2);
+          hipLaunchKernelGGL(k_qmr_upd<R>, g(), b(), 0, st, v[ipo], v[ipn], vv, vt, x, r, mode, dst, n, part);
+          scalar(S_END_ITER);
+          std::swap(ipo, ipn);
+          std::swap(iv, ivt);
+          break;
+        }
+        case DDA_SOLVER_BICGSTAB: {
+          C* p = v[4];
+          C* vv = v[5];
+          C* sv = v[6];
+          C* rt = v[7];
+          reduce(RED_DOTC, r, rt, done);
+          scalar(S_BS_RO);
+          hipLaunchKernelGGL(k_bs_p<R>, g(), b(), 0, st, p, vv, r, nr == 0 ? 1 : 0, dst, n);
+          DDA_TRY(matvec(p, vv, 0, done));
+          reduce(RED_DOTC, vv, rt, done);
+          scalar(S_BS_ALPHA);
+          lincomb(sv, vv, &dst->mAlpha, nullptr, nullptr, r, true, done);  // s = v (-alpha) + r
+          scalar(S_BS_S);
+          hipLaunchKernelGGL(k_bs_early<R>, g(), b(), 0, st, x, p, n, dst);
+          DDA_TRY(matvec(sv, av, 0, skip));
+          hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, av, sv, n, part, dst);
+          scalar(S_BS_OMEGA);
+          hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, n, part, dst);
+          hipLaunchKernelGGL(k_bs_fin, dim3(1), b(), 0, st, dst, part);
+          scalar(S_END_ITER);
+          break;
+        }
+        case DDA_SOLVER_CGNR: {
+          C* p = v[4];
+          DDA_TRY(matvec(r, av, 1, done));  // A^H r
+          reduce(RED_NORM, av, av, done);
+          scalar(S_CG_RO);
+          hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, av, nr == 0 ? 1 : 0, dst, n);
+          DDA_TRY(matvec(p, av, 0, done));
+          reduce(RED_NORM, av, av, done);
+          scalar(S_CG_ALPHA);
+          hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
+          scalar(S_END_ITER);
+          break;
+        }
+        case DDA_SOLVER_BICG_CS: {
+          C* p = v[4];
+          reduce(RED_DOTU, r, r, done);
+          scalar(S_BI_RO);
+          hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, r, nr == 0 ? 1 : 0, dst, n);
+          DDA_TRY(matvec(p, av, 0, done));
+          reduce(RED_DOTU, p, av, done);
+          scalar(S_BI_MU);
+          hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
+          scalar(S_END_ITER);
+          break;
+        }
+      }
+      DDA_TRY_HIP(hipGetLastError());
+      nr++;
+      return DDA_OK;
+    };
+
+    // Chunks of iterations are enqueued ahead of the flag reads: chunk c+1 is
+    // queued before the state copied after chunk c is examined, so the GPU
+    // never waits on the host, and the chunk length follows the observed
+    // convergence rate so that few iterations are queued past the one that
+    // meets the tolerance (those run as no-ops).
+    hipEvent_t ev[2];
+    DDA_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    DDA_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    DDA_TRY_HIP(hipMemcpyAsync(&host2[0], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipEventRecord(ev[0], st));
+    int next_len = 2, chunk = 0;
+    double prev_r = -1;
+    uint64_t prev_count = 0;
+    bool finished = false;
+    dda_status rc = DDA_OK;
+    while (!finished && rc == DDA_OK) {
+      for (int k = 0; k < next_len && rc == DDA_OK; k++) rc = enqueue_iteration();
+      if (rc != DDA_OK) break;
+      const int slot = (chunk + 1) & 1;
+      DDA_TRY_HIP(hipMemcpyAsync(&host2[slot], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+      DDA_TRY_HIP(hipEventRecord(ev[slot], st));
+      const int prev = chunk & 1;
+      DDA_TRY_HIP(hipEventSynchronize(ev[prev]));
+      const SolverState& hs = host2[prev];
+      if (hs.done) {
+        finished = true;
+      } else {
+        int len = kChunk;
+        if (prev_r > 0 && hs.inprodR > 0 && hs.count > prev_count && hs.inprodR < prev_r && hs.epsB > 0) {
+          const double rate = std::log(prev_r / hs.inprodR) / (double)(hs.count - prev_count);
+          const double need = std::log(hs.inprodR / hs.epsB) / rate - (double)next_len;
+          len = need < 1 ? 1 : (need > kChunk ? kChunk : (int)std::ceil(need));
+        }
+        prev_r = hs.inprodR;
+        prev_count = hs.count;
+        next_len = len;
+      }
+      chunk++;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc != DDA_OK) return rc;
+    DDA_TRY_HIP(hipMemcpy(fin, dst, sizeof(SolverState), hipMemcpyDeviceToHost));
+    return DDA_OK;
+  }
+};
+
+template struct Core<float>;
+template struct Core<double>;
+
 }  // namespace dda
 
 using namespace dda;
 
 struct dda_solver_s {
-  dda_op_s* op = nullptr;
+  dda_op_s* op = nullptr;        // FP64 operator (caller's)
+  dda_op_s* op32 = nullptr;      // FP32 twin (mixed mode), owned
   int kind = 0;
+  int mode = DDA_MODE_FP64;
   uint64_t max_iter = 0, max_res_increase = 0;
   long long n = 0;
-  SolverState* st = nullptr;       // device
-  SolverState* st_host = nullptr;  // pinned
-  double* part = nullptr;
-  double* hist = nullptr;
-  uint64_t hist_cap = 0;
-  void* vec[8] = {};               // b, r, x, av, t1, t2, t3, t4
-  double2* io = nullptr;           // FP64 staging for einc / start / result
+  Core<double> c64;              // FP64 mode
+  Core<float> c32;               // mixed mode inner solver
+  double2* x64 = nullptr;        // mixed mode: FP64 b, x, r, t
+  double2* b64 = nullptr;
+  double2* r64 = nullptr;
+  double2* t64 = nullptr;
+  double* part64 = nullptr;
+  double2* io = nullptr;         // FP64 staging for einc / start / result
   uint64_t last_iters = 0;
+  std::vector<double> outer_hist;
+  double inner_floor = 2e-5;
 };
 
 namespace {
 
-template <typename R>
-struct Runner {
-  typedef typename CxT<R>::T C;
-  dda_solver_s* s;
-  hipStream_t st;
-  long long n;
-  const int* done;
-  C* v(int i) { return (C*)s->vec[i]; }
+dda_status true_residual64(dda_solver_s* s, double2* x, double2* b, double* rel2, double temp) {
+  Core<double>& K = s->c64;
+  (void)K;
+  hipStream_t st = s->op->ctx->stream;
+  DDA_TRY(launch_matvec<double>(s->op, x, s->t64, 0, st, nullptr));
+  hipLaunchKernelGGL(k_resid64, dim3(kRedBlocks), dim3(kRedThreads), 0, st, s->r64, b, s->t64, s->n, s->part64);
+  DDA_TRY_HIP(hipGetLastError());
+  std::vector<double> h(kRedBlocks);
+  DDA_TRY_HIP(hipMemcpyAsync(h.data(), s->part64, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+  DDA_TRY_HIP(hipStreamSynchronize(st));
+  double t = 0;
+  for (double v : h) t += v;
+  *rel2 = t / temp;
+  return DDA_OK;
+}
 
-  dim3 g() const { return dim3(kRedBlocks); }
-  dim3 b() const { return dim3(kRedThreads); }
-
-  dda_status matvec(const void* x, void* y, int conj, const int* flag) {
-    return launch_matvec<R>(s->op, x, y, conj, st, flag);
-  }
-  void scalar(int step, double eps = 0) {
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kRedThreads), 0, st, s->st, s->part, step, s->kind, eps, s->hist);
-  }
-  void reduce(int op, const C* a, const C* bb, const int* flag) {
-    if (op == RED_NORM)
-      hipLaunchKernelGGL((k_reduce<R, RED_NORM>), g(), b(), 0, st, a, bb, n, s->part, flag);
-    else if (op == RED_DOTU)
-      hipLaunchKernelGGL((k_reduce<R, RED_DOTU>), g(), b(), 0, st, a, bb, n, s->part, flag);
-    else
-      hipLaunchKernelGGL((k_reduce<R, RED_DOTC>), g(), b(), 0, st, a, bb, n, s->part, flag);
-  }
-  void lincomb(C* out, const C* v1, const double2* c1, const C* v2, const double2* c2, const C* v3, bool part,
-               const int* flag) {
-    hipLaunchKernelGGL(k_lincomb<R>, g(), b(), 0, st, out, v1, c1, v2, c2, v3, n, part ? s->part : nullptr, flag);
-  }
-};
-
-}  // namespace
-
-template <typename R>
-static dda_status run_solve(dda_solver_s* s, double eps, bool have_start, dda_solve_info* info) {
-  typedef typename CxT<R>::T C;
+dda_status run_fp64(dda_solver_s* s, double eps, bool have_start, dda_solve_info* info) {
   dda_op_s* op = s->op;
   hipStream_t st = op->ctx->stream;
-  Runner<R> K{s, st, s->n, &s->st->done};
   const MvDims& d = op->dims;
-  SolverState* dst = s->st;
-  const int* done = &dst->done;
-  const int* skip = &dst->skip;
-  C* b = K.v(0);
-  C* r = K.v(1);
-  C* x = K.v(2);
-  C* av = K.v(3);
-  const C* ccs = (const C*)op->ccs;
-
-  // host-side initial state
-  SolverState h;
-  std::memset(&h, 0, sizeof h);
-  h.max_iter = s->max_iter;
-  h.max_res_increase = s->max_res_increase;
-  h.hist_cap = s->hist_cap;
-  DDA_TRY_HIP(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, st));
-
-  // initGeneral (GpuIterativeSolver.cpp:78-134)
-  hipLaunchKernelGGL(k_setup_b<R>, K.g(), K.b(), 0, st, b, s->io, ccs, op->mat_dev, d.nv, d.S, s->part);
-  K.scalar(S_INIT_TEMP, eps);
-  if (have_start) {
-    const double gu = op->grid_unit;
-    hipLaunchKernelGGL(k_setup_start<R>, K.g(), K.b(), 0, st, x, s->io + s->n, ccs, op->mat_dev, d.nv, d.S,
-                       1.0 / (gu * gu * gu));
-    DDA_TRY(K.matvec(x, av, 0, nullptr));
-    K.lincomb(r, av, &dst->mOne, nullptr, nullptr, b, true, skip);  // r = b - A x0
-    K.scalar(S_INIT_START);
-  } else {
-    DDA_TRY(K.matvec(b, av, 0, nullptr));
-    K.lincomb(r, av, &dst->mOne, nullptr, nullptr, b, true, skip);  // r = b - A b
-    K.scalar(S_INIT_R);
-    hipLaunchKernelGGL(k_select_x0<R>, K.g(), K.b(), 0, st, x, r, b, s->n, dst);
-  }
-  DDA_TRY_HIP(hipGetLastError());
-
-  // solver init
-  if (s->kind == DDA_SOLVER_QMR_CS) {
-    K.reduce(RED_DOTU, r, r, skip);
-    K.scalar(S_QMR_INIT);
-    K.lincomb(K.v(4), r, &dst->betaInv, nullptr, nullptr, nullptr, false, skip);  // v = r / beta
-  } else if (s->kind == DDA_SOLVER_BICGSTAB) {
-    hipLaunchKernelGGL(k_copy_flag<R>, K.g(), K.b(), 0, st, K.v(7), r, s->n, skip);  // rtilda = r
-  }
-  K.scalar(S_LOOP_CHECK);
-
-  // iteration roles (swapped on the host each iteration)
-  int iv = 4, ivt = 5, ipo = 6, ipn = 7;
-  uint64_t nr = 0;
-  bool finished = false;
-  hipEvent_t ev[2];
-  DDA_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  DDA_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int chunk = 0;
-  auto enqueue_iteration = [&]() -> dda_status {
-    switch (s->kind) {
-      case DDA_SOLVER_QMR_CS: {
-        C* vv = K.v(iv);
-        C* vt = K.v(ivt);
-        DDA_TRY(K.matvec(vv, av, 0, done));
-        K.reduce(RED_DOTU, vv, av, done);
-        K.scalar(S_QMR_ALPHA);
-        hipLaunchKernelGGL(k_qmr_vt<R>, K.g(), K.b(), 0, st, vt, vv, av, nr == 0 ? 1 : 0, dst, s->n, s->part);
-        K.scalar(S_QMR_GIVENS);
-        const int mode = nr == 0 ? 0 : (nr == 1 ? 1 : 2);
-        hipLaunchKernelGGL(k_qmr_upd<R>, K.g(), K.b(), 0, st, K.v(ipo), K.v(ipn), vv, vt, x, r, mode, dst, s->n,
-                           s->part);
-        K.scalar(S_END_ITER);
-        std::swap(ipo, ipn);
-        std::swap(iv, ivt);
-        break;
-      }
-      case DDA_SOLVER_BICGSTAB: {
-        C* p = K.v(4);
-        C* v = K.v(5);
-        C* sv = K.v(6);
-        C* rt = K.v(7);
-        K.reduce(RED_DOTC, r, rt, done);
-        K.scalar(S_BS_RO);
-        hipLaunchKernelGGL(k_bs_p<R>, K.g(), K.b(), 0, st, p, v, r, nr == 0 ? 1 : 0, dst, s->n);
-        DDA_TRY(K.matvec(p, v, 0, done));
-        K.reduce(RED_DOTC, v, rt, done);
-        K.scalar(S_BS_ALPHA);
-        K.lincomb(sv, v, &dst->mAlpha, nullptr, nullptr, r, true, done);  // s = v (-alpha) + r
-        K.scalar(S_BS_S);
-        hipLaunchKernelGGL(k_bs_early<R>, K.g(), K.b(), 0, st, x, p, s->n, dst);
-        DDA_TRY(K.matvec(sv, av, 0, skip));
-        hipLaunchKernelGGL(k_bs_om<R>, K.g(), K.b(), 0, st, av, sv, s->n, s->part, dst);
-        K.scalar(S_BS_OMEGA);
-        hipLaunchKernelGGL(k_bs_upd<R>, K.g(), K.b(), 0, st, x, r, p, sv, av, s->n, s->part, dst);
-        hipLaunchKernelGGL(k_bs_fin, dim3(1), K.b(), 0, st, dst, s->part);
-        K.scalar(S_END_ITER);
-        break;
-      }
-      case DDA_SOLVER_CGNR: {
-        C* p = K.v(4);
-        DDA_TRY(K.matvec(r, av, 1, done));  // A^H r
-        K.reduce(RED_NORM, av, av, done);
-        K.scalar(S_CG_RO);
-        hipLaunchKernelGGL(k_p_update<R>, K.g(), K.b(), 0, st, p, av, nr == 0 ? 1 : 0, dst, s->n);
-        DDA_TRY(K.matvec(p, av, 0, done));
-        K.reduce(RED_NORM, av, av, done);
-        K.scalar(S_CG_ALPHA);
-        hipLaunchKernelGGL(k_xr_update<R>, K.g(), K.b(), 0, st, x, r, p, av, s->n, s->part, dst);
-        K.scalar(S_END_ITER);
-        break;
-      }
-      case DDA_SOLVER_BICG_CS: {
-        C* p = K.v(4);
-        K.reduce(RED_DOTU, r, r, done);
-        K.scalar(S_BI_RO);
-        hipLaunchKernelGGL(k_p_update<R>, K.g(), K.b(), 0, st, p, r, nr == 0 ? 1 : 0, dst, s->n);
-        DDA_TRY(K.matvec(p, av, 0, done));
-        K.reduce(RED_DOTU, p, av, done);
-        K.scalar(S_BI_MU);
-        hipLaunchKernelGGL(k_xr_update<R>, K.g(), K.b(), 0, st, x, r, p, av, s->n, s->part, dst);
-        K.scalar(S_END_ITER);
-        break;
-      }
-    }
-    DDA_TRY_HIP(hipGetLastError());
-    nr++;
-    return DDA_OK;
-  };
-
-  // Enqueue chunks of iterations; chunk c+1 is queued before waiting on the
-  // state copied after chunk c, so the GPU never idles on the host. Chunks
-  // queued after the device has finished run as no-ops.
-  // The chunk length follows the observed convergence rate so that few
-  // iterations are queued past the one that meets the tolerance.
-  DDA_TRY_HIP(hipMemcpyAsync(&s->st_host[0], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
-  DDA_TRY_HIP(hipEventRecord(ev[0], st));
-  int next_len = 2;
-  double prev_r = -1;
-  uint64_t prev_count = 0;
-  while (!finished) {
-    for (int k = 0; k < next_len; k++) DDA_TRY(enqueue_iteration());
-    const int slot = (chunk + 1) & 1;
-    DDA_TRY_HIP(hipMemcpyAsync(&s->st_host[slot], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
-    DDA_TRY_HIP(hipEventRecord(ev[slot], st));
-    const int prev = chunk & 1;
-    DDA_TRY_HIP(hipEventSynchronize(ev[prev]));
-    const SolverState& hs = s->st_host[prev];
-    if (hs.done) {
-      finished = true;
-    } else {
-      // iterations still needed at the rate seen since the last read, less
-      // the chunk already queued behind this state
-      int len = kChunk;
-      if (prev_r > 0 && hs.inprodR > 0 && hs.count > prev_count && hs.inprodR < prev_r && hs.epsB > 0) {
-        const double rate = std::log(prev_r / hs.inprodR) / (double)(hs.count - prev_count);
-        const double need = std::log(hs.inprodR / hs.epsB) / rate - (double)next_len;
-        len = need < 1 ? 1 : (need > kChunk ? kChunk : (int)std::ceil(need));
-      }
-      prev_r = hs.inprodR;
-      prev_count = hs.count;
-      next_len = len;
-    }
-    chunk++;
-  }
-  DDA_TRY_HIP(hipStreamSynchronize(st));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  SolverState fin;
-  DDA_TRY_HIP(hipMemcpy(&fin, dst, sizeof fin, hipMemcpyDeviceToHost));
-  s->last_iters = fin.count;
-
-  // getResult (GpuIterativeSolver.cpp:136-144): p = sqrt(a) x, back in m^3 units
+  Core<double>& K = s->c64;
+  const double2* ccs = (const double2*)op->ccs;
   const double gu = op->grid_unit;
-  hipLaunchKernelGGL(k_get_result<R>, K.g(), K.b(), 0, st, s->io, x, ccs, op->mat_dev, d.nv, d.S, gu * gu * gu);
+  hipLaunchKernelGGL(k_setup_b<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, K.v[0], s->io, ccs, op->mat_dev,
+                     d.nv, d.S, K.part);
+  if (have_start)
+    hipLaunchKernelGGL(k_setup_start<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, K.v[2], s->io + s->n, ccs,
+                       op->mat_dev, d.nv, d.S, 1.0 / (gu * gu * gu));
   DDA_TRY_HIP(hipGetLastError());
-
-  // true residual |b - A x|^2 / |b|^2
-  int zero = 0;
-  int* zflag = &dst->sel;  // reuse an int that is 0 or 1; force 0
-  DDA_TRY_HIP(hipMemcpyAsync(zflag, &zero, sizeof zero, hipMemcpyHostToDevice, st));
-  DDA_TRY(K.matvec(x, av, 0, nullptr));
-  K.lincomb(av, av, &dst->mOne, nullptr, nullptr, b, false, zflag);
-  K.reduce(RED_NORM, av, av, zflag);
+  SolverState fin;
+  DDA_TRY(K.solve(eps, have_start, &fin));
+  s->last_iters = fin.count;
+  hipLaunchKernelGGL(k_get_result<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, s->io, K.v[2], ccs,
+                     op->mat_dev, d.nv, d.S, gu * gu * gu);
   DDA_TRY_HIP(hipGetLastError());
-  std::vector<double> parts(kRedBlocks);
-  DDA_TRY_HIP(hipMemcpyAsync(parts.data(), s->part, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
-  DDA_TRY_HIP(hipStreamSynchronize(st));
-  double tr = 0;
-  for (double v : parts) tr += v;
+  double rel2 = 0;
+  DDA_TRY(true_residual64(s, K.v[2], K.v[0], &rel2, fin.temp));
   if (info) {
     info->iterations = fin.count;
     info->rel_residual = std::sqrt(fin.residScale * fin.inprodR);
-    info->true_rel_residual = std::sqrt(tr * fin.residScale);
+    info->true_rel_residual = std::sqrt(rel2);
     info->refinements = 0;
     const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
     info->matvecs = fin.count * per + 2;
@@ -914,49 +997,169 @@ static dda_status run_solve(dda_solver_s* s, double eps, bool have_start, dda_so
   return (dda_status)fin.status;
 }
 
-extern "C" dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
-                                        dda_solver_t* out) {
-  if (!op || !out || kind < 0 || kind > 3) {
-    set_error("dda_solver_create: null argument or unknown solver kind");
+// FP64 iterative refinement around an FP32 Krylov solve:
+//   r = b - A64 x; stop when |r|^2 < eps^2 |b|^2; solve A32 d = r / |r| to
+//   max(floor, eps |b| / (2 |r|)); x += |r| d.
+dda_status run_mixed(dda_solver_s* s, double eps, bool have_start, dda_solve_info* info) {
+  dda_op_s* op = s->op;
+  hipStream_t st = op->ctx->stream;
+  const MvDims& d = op->dims;
+  Core<float>& K = s->c32;
+  const double2* ccs = (const double2*)op->ccs;
+  const double gu = op->grid_unit;
+  const dim3 G(kRedBlocks), B(kRedThreads);
+  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->b64, s->io, ccs, op->mat_dev, d.nv, d.S, s->part64);
+  DDA_TRY_HIP(hipGetLastError());
+  std::vector<double> h(kRedBlocks);
+  DDA_TRY_HIP(hipMemcpyAsync(h.data(), s->part64, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+  DDA_TRY_HIP(hipStreamSynchronize(st));
+  double temp = 0;
+  for (double v : h) temp += v;
+  if (!(temp > 0)) {
+    set_error("zero right-hand side");
+    return DDA_INVALID_ARG;
+  }
+  if (have_start)
+    hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, s->x64, s->io + s->n, ccs, op->mat_dev, d.nv, d.S,
+                       1.0 / (gu * gu * gu));
+  else
+    DDA_TRY_HIP(hipMemsetAsync(s->x64, 0, s->n * sizeof(double2), st));
+  s->outer_hist.clear();
+  uint64_t iters = 0, outer = 0, matvecs = 0;
+  dda_status status = DDA_OK;
+  double rel2 = 1.0;
+  const uint64_t max_outer = 40;
+  for (;;) {
+    if (outer == 0 && !have_start) {
+      DDA_TRY_HIP(hipMemcpyAsync(s->r64, s->b64, s->n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      rel2 = 1.0;
+    } else {
+      DDA_TRY(true_residual64(s, s->x64, s->b64, &rel2, temp));
+      matvecs++;
+    }
+    s->outer_hist.push_back(rel2);
+    if (rel2 < eps * eps) break;
+    if (outer >= max_outer || iters >= s->max_iter) {
+      status = DDA_MAX_ITER;
+      break;
+    }
+    const double rn = std::sqrt(rel2 * temp);
+    hipLaunchKernelGGL(k_to_f32, G, B, 0, st, (float2*)K.v[0], s->r64, s->n, 1.0 / rn);
+    DDA_TRY_HIP(hipGetLastError());
+    const double eps_in = std::max(s->inner_floor, 0.5 * eps / std::sqrt(rel2));
+    K.max_iter = s->max_iter - iters;
+    SolverState fin;
+    DDA_TRY(K.solve(eps_in, false, &fin));
+    iters += fin.count;
+    const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
+    matvecs += fin.count * per + 1;
+    outer++;
+    if (fin.status == DDA_BREAKDOWN && fin.count == 0) {
+      status = DDA_BREAKDOWN;
+      break;
+    }
+    hipLaunchKernelGGL(k_add_f32, G, B, 0, st, s->x64, (const float2*)K.v[2], s->n, rn);
+    DDA_TRY_HIP(hipGetLastError());
+  }
+  s->last_iters = iters;
+  hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, s->x64, ccs, op->mat_dev, d.nv, d.S, gu * gu * gu);
+  DDA_TRY_HIP(hipGetLastError());
+  if (info) {
+    info->iterations = iters;
+    info->rel_residual = std::sqrt(rel2);
+    info->true_rel_residual = std::sqrt(rel2);
+    info->refinements = outer;
+    info->matvecs = matvecs;
+  }
+  return status;
+}
+
+}  // namespace
+
+static void free_solver(dda_solver_s* s) {
+  s->c64.release();
+  s->c32.release();
+  void* bufs[] = {s->x64, s->b64, s->r64, s->t64, s->part64, s->io};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->op32) dda_op_destroy(s->op32);
+  delete s;
+}
+
+extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
+                                           int mode, dda_solver_t* out) {
+  if (!op || !out || kind < 0 || kind > 3 || (mode != DDA_MODE_FP64 && mode != DDA_MODE_MIXED)) {
+    set_error("dda_solver_create: null argument, unknown solver kind or mode");
+    return DDA_INVALID_ARG;
+  }
+  if (op->prec != 64) {
+    set_error("dda_solver_create: the operator must be FP64 (mixed mode builds its own FP32 twin)");
     return DDA_INVALID_ARG;
   }
   *out = nullptr;
   dda_solver_s* s = new dda_solver_s();
   s->op = op;
   s->kind = kind;
+  s->mode = mode;
   const MvDims& d = op->dims;
   s->max_iter = max_iter ? max_iter : 3ull * d.Bx * d.By * d.Bz;  // DDA/DDAMain.cpp:640-644
   static const uint64_t defaults[4] = {50000, 30000, 10, 50000};  // QmrCs.cpp:39, BicgStab.cpp:51, Cgnr.cpp:34
   s->max_res_increase = max_res_increase ? max_res_increase : defaults[kind];
   s->n = 3ll * d.S;
-  s->hist_cap = std::min<uint64_t>(s->max_iter + 1, 1u << 20);
-  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
-  hipError_t e = hipMalloc(&s->st, sizeof(SolverState));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&s->st_host, 2 * sizeof(SolverState), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc(&s->part, 4 * kRedBlocks * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&s->hist, s->hist_cap * sizeof(double));
-  for (int i = 0; i < 8 && e == hipSuccess; i++) e = hipMalloc(&s->vec[i], s->n * csz);
-  if (e == hipSuccess) e = hipMalloc(&s->io, 2 * s->n * sizeof(double2));
-  if (e != hipSuccess) {
-    set_error(std::string("dda_solver_create: ") + hipGetErrorString(e));
-    dda_solver_destroy(s);
-    return e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+  dda_status rc = DDA_OK;
+  auto dm = [&](void** p, size_t bytes) {
+    if (rc != DDA_OK) return;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      set_error(std::string("dda_solver_create: ") + hipGetErrorString(e));
+      rc = e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+    }
+  };
+  dm((void**)&s->io, 2 * s->n * sizeof(double2));
+  dm((void**)&s->t64, s->n * sizeof(double2));
+  dm((void**)&s->r64, s->n * sizeof(double2));
+  dm((void**)&s->part64, 4 * kRedBlocks * sizeof(double));
+  if (rc == DDA_OK && mode == DDA_MODE_FP64) rc = s->c64.alloc(op, kind, s->max_iter, s->max_res_increase);
+  if (rc == DDA_OK && mode == DDA_MODE_MIXED) {
+    dm((void**)&s->x64, s->n * sizeof(double2));
+    dm((void**)&s->b64, s->n * sizeof(double2));
+    if (rc == DDA_OK) {
+      dda_op_desc desc;
+      std::memset(&desc, 0, sizeof desc);
+      desc.box[0] = d.Bx;
+      desc.box[1] = d.By;
+      desc.box[2] = d.Bz;
+      desc.grid[0] = d.Gx;
+      desc.grid[1] = d.Gy;
+      desc.grid[2] = d.Gz;
+      desc.nv = d.nv;
+      desc.vec_stride = d.S;
+      desc.pos = op->pos.data();
+      desc.mat = op->mat.data();
+      desc.nmat = op->nmat;
+      desc.grid_unit = op->grid_unit;
+      desc.wave_num = op->wave_num;
+      rc = dda_op_create(op->ctx, &desc, 32, &s->op32);
+    }
+    if (rc == DDA_OK) rc = s->c32.alloc(s->op32, kind, s->max_iter, s->max_res_increase);
+  }
+  if (rc != DDA_OK) {
+    free_solver(s);
+    return rc;
   }
   *out = s;
   return DDA_OK;
 }
 
+extern "C" dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
+                                        dda_solver_t* out) {
+  return dda_solver_create_ex(op, kind, max_iter, max_res_increase, DDA_MODE_FP64, out);
+}
+
 extern "C" dda_status dda_solver_destroy(dda_solver_t s) {
   if (!s) return DDA_OK;
   if (s->op) (void)hipStreamSynchronize(s->op->ctx->stream);
-  if (s->st) (void)hipFree(s->st);
-  if (s->st_host) (void)hipHostFree(s->st_host);
-  if (s->part) (void)hipFree(s->part);
-  if (s->hist) (void)hipFree(s->hist);
-  for (void* v : s->vec)
-    if (v) (void)hipFree(v);
-  if (s->io) (void)hipFree(s->io);
-  delete s;
+  free_solver(s);
   return DDA_OK;
 }
 
@@ -973,10 +1176,14 @@ extern "C" dda_status dda_solver_solve(dda_solver_t s, const double* einc, doubl
   }
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = op->ctx->stream;
+  if (s->mode == DDA_MODE_MIXED) {
+    DDA_TRY(dda_op_set_cc_sqrt(s->op32, op->cc_sqrt_host.data()));
+    s->op32->prof = op->prof;
+  }
   DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, s->n * sizeof(double2), hipMemcpyHostToDevice, st));
   if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + s->n, start, s->n * sizeof(double2), hipMemcpyHostToDevice, st));
-  dda_status rc = op->prec == 64 ? run_solve<double>(s, eps, start != nullptr, info)
-                                 : run_solve<float>(s, eps, start != nullptr, info);
+  dda_status rc = s->mode == DDA_MODE_MIXED ? run_mixed(s, eps, start != nullptr, info)
+                                            : run_fp64(s, eps, start != nullptr, info);
   if (rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_INVALID_ARG || rc == DDA_NOT_SUPPORTED) return rc;
   DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, s->n * sizeof(double2), hipMemcpyDeviceToHost, st));
   DDA_TRY_HIP(hipStreamSynchronize(st));
@@ -992,8 +1199,31 @@ extern "C" dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n
     set_error("dda_solver_history: null argument");
     return DDA_INVALID_ARG;
   }
-  uint64_t m = std::min<uint64_t>(std::min<uint64_t>(n, s->last_iters + 1), s->hist_cap);
-  if (m) DDA_TRY_HIP(hipMemcpy(out, s->hist, m * sizeof(double), hipMemcpyDeviceToHost));
+  uint64_t m = 0;
+  if (s->mode == DDA_MODE_MIXED) {
+    m = std::min<uint64_t>(n, s->outer_hist.size());
+    for (uint64_t i = 0; i < m; i++) out[i] = s->outer_hist[i];
+  } else {
+    m = std::min<uint64_t>(std::min<uint64_t>(n, s->last_iters + 1), s->c64.hist_cap);
+    if (m) DDA_TRY_HIP(hipMemcpy(out, s->c64.hist, m * sizeof(double), hipMemcpyDeviceToHost));
+  }
   if (written) *written = m;
   return DDA_OK;
+}
+
+extern "C" dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_) {
+  if (!s || !(floor_ > 0 && floor_ < 1)) {
+    set_error("dda_solver_set_inner_floor: floor must be in (0, 1)");
+    return DDA_INVALID_ARG;
+  }
+  s->inner_floor = floor_;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_solver_profile_read(dda_solver_t s, double ms[5], uint64_t counts[5], uint64_t* skipped) {
+  if (!s) {
+    set_error("dda_solver_profile_read: null solver");
+    return DDA_INVALID_ARG;
+  }
+  return dda_op_profile_read(s->mode == DDA_MODE_MIXED ? s->op32 : s->op, ms, counts, skipped);
 }

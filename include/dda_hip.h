@@ -138,12 +138,32 @@ typedef struct dda_solve_info {
   double seconds;        /* wall time of the solve */
 } dda_solve_info;
 
-/* new GpuQmrCs(...) etc. (DDA/DDAMain.cpp:647-656). op must be an FP64
- * operator. max_iter = 0 selects 3*Bx*By*Bz (DDA/DDAMain.cpp:640-644);
+typedef enum dda_solver_mode {
+  DDA_MODE_FP64 = 0,  /* every operation in FP64, as the reference's double path */
+  DDA_MODE_MIXED = 1  /* FP32 Krylov solves (FP64 scalars) inside FP64 iterative refinement */
+} dda_solver_mode;
+
+/* new GpuQmrCs(...) etc. (DDA/DDAMain.cpp:647-656) in FP64 mode; op must be an
+ * FP64 operator. max_iter = 0 selects 3*Bx*By*Bz (DDA/DDAMain.cpp:640-644);
  * max_res_increase = 0 selects the solver's reference default (QMR 50000,
  * BiCGStab 30000, CGNR 10, BiCG 50000). */
 dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
                              dda_solver_t* out);
+/* As dda_solver_create with a mode. DDA_MODE_MIXED has no counterpart in the
+ * reference: it builds an FP32 twin of op (tensor generated in FP64), runs the
+ * Krylov iteration in FP32 with FP64 scalars and reductions, and refines in
+ * FP64: r = b - A x, solve A d = r / |r| in FP32 to max(floor, eps |b| / 2|r|),
+ * x += |r| d, until the FP64 residual meets eps. max_iter bounds the summed
+ * inner iterations; the history then holds the FP64 residuals of the outer
+ * steps. */
+dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase, int mode,
+                                dda_solver_t* out);
+/* Tolerance floor of the FP32 inner solves in mixed mode (default 2e-5). */
+dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
+/* dda_op_profile_read for the operator the Krylov iteration runs on: the FP32
+ * twin in mixed mode (it follows the FP64 operator's dda_op_profile switch),
+ * else the FP64 operator itself. */
+dda_status dda_solver_profile_read(dda_solver_t s, double ms[5], uint64_t counts[5], uint64_t* skipped);
 dda_status dda_solver_destroy(dda_solver_t s);
 /* IterativeSolverBase::getPolVec, DDA/IterativeSolverBase.cpp:40-103.
  * einc: host [3*vec_stride] complex double; start (nullable) is a previous

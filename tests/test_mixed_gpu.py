@@ -1,0 +1,54 @@
+"""FP32-inner / FP64-outer refinement: same answer as the FP64 oracle to the
+requested residual, Cext within 1e-5 of the FP64 solve."""
+import numpy as np
+import pytest
+
+import dda_amd as D
+from dda_amd import problem as P
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _op(ctx, g, params, cs):
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=64, grid=params.grid,
+                     nmat=len(g.materials))
+    op.set_couple_constants(cs)
+    return op
+
+
+@pytest.mark.parametrize("kind", ["qmr", "bicgstab", "cgnr", "bicg"])
+@pytest.mark.parametrize("name", ["sphere_r7", "two_materials"])
+def test_mixed_matches_fp64_oracle(ctx, kind, name):
+    print("mixed", kind, name, flush=True)
+    g, params, cc, cs, prop, pol = cases.build(name)
+    einc = P.einc_plane(params, prop, pol)
+    orc = cases.oracle_problem(g, params, cs)
+    st, p_ref, it_ref, _ = orc.solve(kind, einc, 1e-11, max_iter=5000)
+    assert st == 0
+    s = D.SOLVER_CLASSES[kind](_op(ctx, g, params, cs), max_iter=5000, mixed=True)
+    eps = 1e-6
+    p = s.get_pol_vec(einc, eps)
+    info = s.info
+    assert info["true_rel_residual"] < eps
+    assert info["refinements"] >= 1
+    # the residual bound eps limits the solution error to about eps * cond(A)
+    assert cases.rel_err(p, p_ref) <= 1e-4
+    cext, cext_ref = P.ext_cross(params, p, einc), orc.ext_cross_sum(p_ref, einc)
+    assert abs(cext - cext_ref) <= 1e-5 * abs(cext_ref)
+    h = s.history()
+    assert len(h) == info["refinements"] + 1 and h[-1] < eps * eps
+
+
+def test_mixed_c1_to_1e8(ctx):
+    """C1 (40 across, 80^3) to 1e-8 in mixed precision: several refinement steps."""
+    g, params, cc, cs, prop, pol = cases.build("sphere_c1")
+    einc = P.einc_plane(params, prop, pol)
+    op = _op(ctx, g, params, cs)
+    p64 = D.GpuQmrCs(op, max_iter=3000).get_pol_vec(einc, 1e-10)
+    s = D.GpuQmrCs(op, max_iter=3000, mixed=True)
+    p = s.get_pol_vec(einc, 1e-8)
+    assert s.info["true_rel_residual"] < 1e-8 and s.info["refinements"] >= 2
+    assert abs(P.ext_cross(params, p, einc) / P.ext_cross(params, p64, einc) - 1) < 1e-7
+    assert abs(P.abs_cross(params, p, cc) / P.abs_cross(params, p64, cc) - 1) < 1e-7
