@@ -72,11 +72,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--solver", default="qmr", choices=["qmr", "bicgstab", "cgnr", "bicg"])
+    ap.add_argument("--runs", default="fp64:qmr,mixed:qmr,mixed:bicgstab",
+                    help="mode:solver pairs to time; value = the last one, the FP64 QMR-CS run is the baseline")
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--modes", default="fp64,mixed", help="solver modes to time; value = the last one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU slab "
@@ -105,10 +105,11 @@ def main():
     t0 = time.perf_counter()
     op = D.GpuMatVec(ctx, g.pos, g.mat, box, d, params.wave_num, precision=64, grid=grid)
     t_setup = time.perf_counter() - t0
-    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    modes = [m.strip() for m in args.runs.split(",") if m.strip()]
 
-    def run_mode(mode):
-        solver = D.SOLVER_CLASSES[args.solver](op, max_iter=20000, mixed=(mode == "mixed"))
+    def run_mode(run):
+        mode, kind = run.split(":")
+        solver = D.SOLVER_CLASSES[kind](op, max_iter=20000, mixed=(mode == "mixed"))
 
         def step():
             out = []
@@ -138,9 +139,10 @@ def main():
     runs = {}
     for mode in modes:
         runs[mode] = run_mode(mode)
-    final_mode = modes[-1]
-    results, elapsed, _ = runs[final_mode]
-    ms, cnt, skipped = runs[final_mode][2]
+    final_run = modes[-1]
+    final_mode, final_solver = final_run.split(":")
+    results, elapsed, _ = runs[final_run]
+    ms, cnt, skipped = runs[final_run][2]
 
     nsolves = 2 * args.steps
     value = nsolves / elapsed
@@ -169,7 +171,7 @@ def main():
     for mode, (res, el, prof) in runs.items():
         o = observe(res)
         pms, pcnt, _ = prof
-        cm = 8 if mode == "mixed" else 16
+        cm = 8 if mode.startswith("mixed") else 16
         pbm = pass_bytes(nv, S, box, grid, rows, cm)
         pav = [pms[i] / pcnt[i] if pcnt[i] else float("nan") for i in range(5)]
         modes_out[mode] = {"solves_per_s": nsolves / el, "ms_per_step": 1e3 * el / args.steps,
@@ -181,12 +183,15 @@ def main():
                            "refinements": [x["refinements"] for x in o],
                            "true_rel_residual": [x["true_rel_residual"] for x in o],
                            "cext": [x["cext"] for x in o], "cabs": [x["cabs"] for x in o]}
-    if "fp64" in modes_out and "mixed" in modes_out:
-        f, m_ = modes_out["fp64"], modes_out["mixed"]
-        modes_out["mixed_vs_fp64"] = {
-            "speedup": m_["solves_per_s"] / f["solves_per_s"],
-            "cext_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cext"], f["cext"])),
-            "cabs_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cabs"], f["cabs"]))}
+    if "fp64:qmr" in modes_out:
+        f = modes_out["fp64:qmr"]
+        for k in list(modes_out):
+            if k.startswith("mixed"):
+                m_ = modes_out[k]
+                m_["vs_fp64_qmr"] = {
+                    "speedup": m_["solves_per_s"] / f["solves_per_s"],
+                    "cext_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cext"], f["cext"])),
+                    "cabs_rel_err": max(abs(a / b - 1) for a, b in zip(m_["cabs"], f["cabs"]))}
     mie = None
     try:
         import oracle as O
@@ -208,12 +213,13 @@ def main():
             t0 = time.perf_counter()
             orc.matvec(x)
             t_mv = time.perf_counter() - t0
-            mv_per_solve = obs[0]["iterations"] + 2
+            fq = modes_out.get("fp64:qmr", {}).get("iterations", [obs[0]["iterations"]])
+            mv_per_solve = fq[0] + 2
             model, ncpu = cpu_info()
             cpu = {"value": 1.0 / (t_mv * mv_per_solve), "unit": "solves/s", "cores": 1, "kind": "port",
                    "sample": "one oracle FFT matvec (MatVecCpu restated, FP64, own FFT) at this workload, "
                              "%.1f s on one core of %s (%d logical CPUs), scaled by the %d matvecs of one "
-                             "QMR-CS solve; vector updates not counted" % (t_mv, model, ncpu, mv_per_solve)}
+                             "FP64 QMR-CS solve; vector updates not counted" % (t_mv, model, ncpu, mv_per_solve)}
         except Exception as e:
             cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": "failed: %s" % e}
 
@@ -233,8 +239,11 @@ def main():
         "data": "synthetic (generated sphere geometry, plane-wave incident field)",
         "config": {"workload": "sphere %d dipoles across (nv=%d), m=1.5+0.1i, |m|kd=0.5, grid %s, prop z, "
                                "2 polarisations, %s to resid %g, %s" % (args.diameter, nv, "x".join(map(str, grid)),
-                                                                      args.solver, args.eps, final_mode),
-                   "nv": nv, "grid": list(grid), "solver": args.solver, "mode": final_mode, "eps": args.eps},
+                                                                      final_solver, args.eps,
+                                                                      "FP32 Krylov in FP64 refinement"
+                                                                      if final_mode == "mixed" else "FP64"),
+                   "nv": nv, "grid": list(grid), "solver": final_solver, "mode": final_mode, "eps": args.eps,
+                   "baseline": "fp64:qmr (the reference's default solver, all FP64)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": PASS_NAMES[dom],
                      "bytes_per_launch": pb[dom], "avg_launch_us": 1e3 * avg_ms[dom],

@@ -32,6 +32,16 @@ def _compile(src):
     return obj, out
 
 
+def build_adapter_check():
+    """The C++ adapter driver used by the GPU tests (g++ against the header)."""
+    root = os.path.dirname(HERE)
+    out = os.path.join(BUILD, "adapter_check")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", out, os.path.join(root, "tests", "cpp", "adapter_check.cpp"),
+           "-L" + os.path.dirname(LIB), "-ldda_hip", "-Wl,-rpath,$ORIGIN/../_lib"]
+    subprocess.check_call(cmd)
+    return out
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
@@ -44,6 +54,7 @@ def build(verbose=False):
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_adapter_check()
     return LIB
 
 

@@ -944,7 +944,7 @@ struct dda_solver_s {
   double2* io = nullptr;         // FP64 staging for einc / start / result
   uint64_t last_iters = 0;
   std::vector<double> outer_hist;
-  double inner_floor = 2e-5;
+  double inner_floor = 1e-4;
 };
 
 namespace {

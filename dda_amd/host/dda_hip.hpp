@@ -1,0 +1,246 @@
+// C++ adapters over the C ABI (include/dda_hip.h) with the shapes of the
+// reference's classes, so that the reference's callers (DDA/DDAMain.cpp ddaCl,
+// createResOutput) can swap the OpenCL path for this one:
+//
+//   Hip::GpuMatVec<T>         GpuMatVec<T>             DDA/GpuMatVec.hpp:85-99
+//   Hip::MatVecHip<T>         MatVec<T> / MatVecGpu<T> DDA/MatVec.hpp:33-55, DDA/MatVecGpu.cpp:51-68
+//   Hip::IterativeSolverBase  IterativeSolverBase<T>   DDA/IterativeSolverBase.hpp:59-62
+//   Hip::GpuQmrCs, GpuBicgStab, GpuCgnr, GpuBicgCs     DDA/GpuQmrCs.hpp:81 etc.
+//
+// Status codes become exceptions (Hip::Error), as the reference's ASSERT /
+// ABORT_MSG become Core::AssertionFailure (Core/Assert.hpp:42-64). Header-only,
+// C++17, no Boost; link with libdda_hip.so.
+#pragma once
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <ostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dda_hip.h"
+
+namespace DDA {
+namespace Hip {
+
+class Error : public std::runtime_error {
+ public:
+  Error(dda_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+  dda_status status;
+};
+
+inline void check(dda_status s, const char* where) {
+  if (s != DDA_OK) throw Error(s, std::string(where) + ": " + dda_last_error());
+}
+
+// One GPU and its in-order stream (the reference's cl::CommandQueue list).
+class Context {
+ public:
+  explicit Context(int device = 0) { check(dda_ctx_create(device, &h_), "dda_ctx_create"); }
+  ~Context() { dda_ctx_destroy(h_); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  dda_ctx_t handle() const { return h_; }
+  void finish() { check(dda_ctx_synchronize(h_), "dda_ctx_synchronize"); }
+
+ private:
+  dda_ctx_t h_ = nullptr;
+};
+
+// The parts of DDAParams / DipoleGeometry the matvec reads.
+struct Problem {
+  std::array<uint32_t, 3> box{};
+  std::array<uint32_t, 3> grid{};           // 0 = fftFit smooth rule
+  std::vector<std::array<uint32_t, 3>> positions;  // DipoleGeometry::positions()
+  std::vector<uint8_t> materials;           // material index per dipole
+  uint32_t matCount = 1;
+  double gridUnit = 0;                      // DDAParams::gridUnit
+  double waveNum = 0;                       // DDAParams::waveNum
+  uint32_t vecStride() const { return (uint32_t)((positions.size() + 15) / 16 * 16); }
+  size_t vecSize() const { return 3 * (size_t)vecStride(); }
+};
+
+// Device buffer of a dipole vector (DipVector<T>).
+template <typename T>
+class DeviceVector {
+ public:
+  DeviceVector(Context& ctx, size_t n) : ctx_(ctx), n_(n) {
+    check(dda_malloc(ctx.handle(), n * sizeof(std::complex<T>), &p_), "dda_malloc");
+  }
+  ~DeviceVector() { dda_free(ctx_.handle(), p_); }
+  DeviceVector(const DeviceVector&) = delete;
+  DeviceVector& operator=(const DeviceVector&) = delete;
+  void write(const std::vector<std::complex<T>>& h) {
+    if (h.size() != n_) throw Error(DDA_INVALID_ARG, "DeviceVector::write: size mismatch");
+    check(dda_memcpy_h2d(ctx_.handle(), p_, h.data(), n_ * sizeof(std::complex<T>)), "dda_memcpy_h2d");
+  }
+  std::vector<std::complex<T>> read() const {
+    std::vector<std::complex<T>> h(n_);
+    check(dda_memcpy_d2h(ctx_.handle(), h.data(), p_, n_ * sizeof(std::complex<T>)), "dda_memcpy_d2h");
+    return h;
+  }
+  void* data() { return p_; }
+  const void* data() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  Context& ctx_;
+  size_t n_;
+  void* p_ = nullptr;
+};
+
+template <typename T> struct Precision;
+template <> struct Precision<double> { static constexpr int value = 64; };
+template <> struct Precision<float> { static constexpr int value = 32; };
+
+// GpuMatVec<T>::create + setCoupleConstants + apply.
+template <typename T>
+class GpuMatVec {
+ public:
+  // dmatrix: optional host D matrix in the reference layout
+  // (SymMatrix3<complex<double>>[Gy][Gz][Gx], Fortran order); null = generate
+  // on the device.
+  GpuMatVec(Context& ctx, const Problem& p, const std::complex<double>* dmatrix = nullptr) : ctx_(ctx), p_(p) {
+    dda_op_desc d{};
+    for (int a = 0; a < 3; a++) {
+      d.box[a] = p.box[a];
+      d.grid[a] = p.grid[a];
+    }
+    d.nv = (uint32_t)p.positions.size();
+    d.vec_stride = p.vecStride();
+    d.pos = p.positions.empty() ? nullptr : p.positions[0].data();
+    d.mat = p.materials.empty() ? nullptr : p.materials.data();
+    d.nmat = p.matCount;
+    d.grid_unit = p.gridUnit;
+    d.wave_num = p.waveNum;
+    d.dmatrix = reinterpret_cast<const double*>(dmatrix);
+    check(dda_op_create(ctx.handle(), &d, Precision<T>::value, &h_), "GpuMatVec::create");
+    uint32_t g[3], vs;
+    size_t bytes;
+    check(dda_op_info(h_, g, &vs, &bytes), "dda_op_info");
+    grid_ = {g[0], g[1], g[2]};
+    deviceBytes_ = bytes;
+  }
+  ~GpuMatVec() { dda_op_destroy(h_); }
+  GpuMatVec(const GpuMatVec&) = delete;
+  GpuMatVec& operator=(const GpuMatVec&) = delete;
+
+  // sqrt of the couple constants per material and axis (CoupleConstants::cc_sqrt()).
+  void setCoupleConstants(const std::vector<std::array<std::complex<double>, 3>>& ccSqrt) {
+    if (ccSqrt.size() != p_.matCount) throw Error(DDA_INVALID_ARG, "setCoupleConstants: one entry per material");
+    check(dda_op_set_cc_sqrt(h_, reinterpret_cast<const double*>(ccSqrt.data())), "setCoupleConstants");
+    cc_ = ccSqrt;
+  }
+  void apply(const DeviceVector<T>& arg, DeviceVector<T>& result, bool conj, void* stream = nullptr) {
+    check(dda_op_apply(h_, arg.data(), result.data(), conj ? 1 : 0, stream), "GpuMatVec::apply");
+  }
+  const Problem& ddaParams() const { return p_; }
+  const std::array<uint32_t, 3>& grid() const { return grid_; }
+  size_t deviceBytes() const { return deviceBytes_; }
+  dda_op_t handle() const { return h_; }
+  Context& context() { return ctx_; }
+
+ private:
+  Context& ctx_;
+  Problem p_;
+  dda_op_t h_ = nullptr;
+  std::array<uint32_t, 3> grid_{};
+  size_t deviceBytes_ = 0;
+  std::vector<std::array<std::complex<double>, 3>> cc_;
+};
+
+// MatVec<T> on host vectors: apply(arg, result, conj), as MatVecGpu<T>.
+template <typename T>
+class MatVecHip {
+ public:
+  explicit MatVecHip(GpuMatVec<T>& op) : op_(op) {}
+  void apply(const std::vector<std::complex<double>>& arg, std::vector<std::complex<double>>& result, bool conj) {
+    const size_t n = op_.ddaParams().vecSize();
+    if (arg.size() != n) throw Error(DDA_INVALID_ARG, "MatVecHip::apply: argument size");
+    result.resize(n);
+    check(dda_op_apply_host(op_.handle(), reinterpret_cast<const double*>(arg.data()),
+                            reinterpret_cast<double*>(result.data()), conj ? 1 : 0),
+          "MatVecHip::apply");
+  }
+  void setCoupleConstants(const std::vector<std::array<std::complex<double>, 3>>& ccSqrt) {
+    op_.setCoupleConstants(ccSqrt);
+  }
+
+ private:
+  GpuMatVec<T>& op_;
+};
+
+// IterativeSolverBase<T>::getPolVec over an FP64 GpuMatVec.
+class IterativeSolverBase {
+ public:
+  virtual ~IterativeSolverBase() { dda_solver_destroy(h_); }
+  IterativeSolverBase(const IterativeSolverBase&) = delete;
+  IterativeSolverBase& operator=(const IterativeSolverBase&) = delete;
+
+  void setCoupleConstants(const std::vector<std::array<std::complex<double>, 3>>& ccSqrt) {
+    op_.setCoupleConstants(ccSqrt);
+  }
+  // einc: incident field SoA [3][vecStride]; returns p = sqrt(a) x. Stops at
+  // |r| < eps |b|; throws Error with DDA_MAX_ITER / DDA_STAGNATION /
+  // DDA_BREAKDOWN where the reference aborts. A residual line is written to
+  // `log` as the reference does at the end of each solve.
+  std::shared_ptr<std::vector<std::complex<double>>> getPolVec(const std::vector<std::complex<double>>& einc,
+                                                               double eps, std::ostream& log,
+                                                               const std::vector<std::complex<double>>& start = {}) {
+    const size_t n = op_.ddaParams().vecSize();
+    if (einc.size() != n || (!start.empty() && start.size() != n))
+      throw Error(DDA_INVALID_ARG, "getPolVec: vector size");
+    auto out = std::make_shared<std::vector<std::complex<double>>>(n);
+    dda_status s = dda_solver_solve(h_, reinterpret_cast<const double*>(einc.data()), eps,
+                                    start.empty() ? nullptr : reinterpret_cast<const double*>(start.data()),
+                                    reinterpret_cast<double*>(out->data()), &info_);
+    log << "iterations = " << info_.iterations << ", |r|/|b| = " << info_.rel_residual << std::endl;
+    check(s, "getPolVec");
+    return out;
+  }
+  const dda_solve_info& lastInfo() const { return info_; }
+  std::vector<double> history() const {
+    std::vector<double> h(info_.iterations + 1);
+    uint64_t w = 0;
+    check(dda_solver_history(h_, h.data(), h.size(), &w), "history");
+    h.resize(w);
+    return h;
+  }
+
+ protected:
+  IterativeSolverBase(GpuMatVec<double>& op, int kind, uint64_t maxIter, bool mixed) : op_(op) {
+    check(dda_solver_create_ex(op.handle(), kind, maxIter, 0, mixed ? DDA_MODE_MIXED : DDA_MODE_FP64, &h_),
+          "solver create");
+  }
+  GpuMatVec<double>& op_;
+  dda_solver_t h_ = nullptr;
+  dda_solve_info info_{};
+};
+
+// new GpuQmrCs<T>(pool, queues, ddaParams, matVec, maxIter, accounting) etc.
+class GpuQmrCs : public IterativeSolverBase {
+ public:
+  GpuQmrCs(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase(op, DDA_SOLVER_QMR_CS, maxIter, mixed) {}
+};
+class GpuBicgStab : public IterativeSolverBase {
+ public:
+  GpuBicgStab(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase(op, DDA_SOLVER_BICGSTAB, maxIter, mixed) {}
+};
+class GpuCgnr : public IterativeSolverBase {
+ public:
+  GpuCgnr(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase(op, DDA_SOLVER_CGNR, maxIter, mixed) {}
+};
+class GpuBicgCs : public IterativeSolverBase {
+ public:
+  GpuBicgCs(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase(op, DDA_SOLVER_BICG_CS, maxIter, mixed) {}
+};
+
+}  // namespace Hip
+}  // namespace DDA

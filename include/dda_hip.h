@@ -158,7 +158,7 @@ dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t 
  * steps. */
 dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase, int mode,
                                 dda_solver_t* out);
-/* Tolerance floor of the FP32 inner solves in mixed mode (default 2e-5). */
+/* Tolerance floor of the FP32 inner solves in mixed mode (default 1e-4). */
 dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
 /* dda_op_profile_read for the operator the Krylov iteration runs on: the FP32
  * twin in mixed mode (it follows the FP64 operator's dda_op_profile switch),
