@@ -55,6 +55,19 @@ def b_mv(nv, box, grid, c):
     return c * (9 * nv + 12 * gx * by * bz + 12 * gx * by * gz + 6 * gx * gy * gz)
 
 
+def measured_traffic(diameter, grid, prec, pass_index):
+    """HBM bytes per launch of one matvec kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size),
+    or None when no measurement exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_matvec_sphere128.md")
+    if diameter != 128 or tuple(grid) != (256, 256, 256) or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    blob = text.split("```json", 1)[1].split("```", 1)[0]
+    data = json.loads(blob)
+    return data[prec][pass_index]["traffic_bytes"]
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -67,6 +80,31 @@ def cpu_info():
     return model, os.cpu_count()
 
 
+def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np):
+    """Copy 512 MiB device to device once, then k matvecs in FP32 and in FP64."""
+    n = (512 << 20) // 16
+    a, b = D.DeviceVector(ctx, n, 64), D.DeviceVector(ctx, n, 64)
+    b.copy_from(a)
+    out = {"copy_bytes": 512 << 20}
+    op32 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=32, grid=params.grid)
+    for name, op in (("fp32", op32), ("fp64", op64)):
+        op.set_couple_constants(rhs[0][2])
+        prec = 32 if name == "fp32" else 64
+        x = D.DeviceVector(ctx, params.vec_size, prec)
+        y = D.DeviceVector(ctx, params.vec_size, prec)
+        x.upload(P.random_dipole_vector(params))
+        op.apply(x, y)
+        ctx.synchronize()
+        op.profile(True)
+        for _ in range(k):
+            op.apply(x, y)
+        ms, cnt, _ = op.profile_read()
+        op.profile(False)
+        out[name + "_pass_avg_us"] = [1e3 * ms[i] / max(cnt[i], 1) for i in range(5)]
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +115,9 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
+                    help="instead of solves: one 512 MiB device copy (counter calibration), then K FP32 and "
+                         "K FP64 matvecs; prints per-pass times (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU slab "
@@ -106,6 +147,8 @@ def main():
     op = D.GpuMatVec(ctx, g.pos, g.mat, box, d, params.wave_num, precision=64, grid=grid)
     t_setup = time.perf_counter() - t0
     modes = [m.strip() for m in args.runs.split(",") if m.strip()]
+    if args.matvec_profile:
+        return matvec_profile(ctx, op, g, params, rhs, args.matvec_profile, D, P, np)
 
     def run_mode(run):
         mode, kind = run.split(":")
@@ -245,7 +288,11 @@ def main():
                    "nv": nv, "grid": list(grid), "solver": final_solver, "mode": final_mode, "eps": args.eps,
                    "baseline": "fp64:qmr (the reference's default solver, all FP64)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": PASS_NAMES[dom],
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.diameter, grid, "fp32" if c == 8 else "fp64", dom),
+                     "traffic_source": "profiles/traffic_matvec_sphere128.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "calibrated)",
+                     "kernel": PASS_NAMES[dom],
                      "bytes_per_launch": pb[dom], "avg_launch_us": 1e3 * avg_ms[dom],
                      "launches": int(cnt[dom]), "noop_launches_excluded": skipped,
                      "measured_in": "%s timed region (%s kernels)" % (final_mode,

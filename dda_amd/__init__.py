@@ -32,7 +32,7 @@ SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_
 # Every symbol include/dda_hip.h declares.
 EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
-    "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_fft_fit",
+    "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_apply",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
@@ -89,6 +89,7 @@ def lib():
         L.dda_free.argtypes = [vp, vp]
         L.dda_memcpy_h2d.argtypes = [vp, vp, vp, sz]
         L.dda_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+        L.dda_memcpy_d2d.argtypes = [vp, vp, vp, sz]
         L.dda_fft_fit.argtypes = [u32, u32, i32]
         L.dda_fft_fit.restype = u32
         L.dda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), i32, pp]
@@ -210,6 +211,11 @@ class DeviceVector:
         h = np.empty(self.n, self.dtype)
         _check(lib().dda_memcpy_d2h(self.ctx.handle, h.ctypes.data, self.ptr, self.nbytes))
         return h
+
+    def copy_from(self, other):
+        """Device-to-device copy of another vector of the same size."""
+        assert other.nbytes == self.nbytes
+        _check(lib().dda_memcpy_d2d(self.ctx.handle, self.ptr, other.ptr, self.nbytes))
 
     def close(self):
         if self.ptr and self.ctx.handle:

@@ -114,3 +114,13 @@ extern "C" dda_status dda_memcpy_d2h(dda_ctx_t ctx, void* dst, const void* src, 
   DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
   return DDA_OK;
 }
+
+extern "C" dda_status dda_memcpy_d2d(dda_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) {
+    set_error("dda_memcpy_d2d: null argument");
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  return DDA_OK;
+}

@@ -70,6 +70,7 @@ dda_status dda_malloc(dda_ctx_t ctx, size_t bytes, void** out);
 dda_status dda_free(dda_ctx_t ctx, void* ptr);
 dda_status dda_memcpy_h2d(dda_ctx_t ctx, void* dst, const void* src, size_t bytes);
 dda_status dda_memcpy_d2h(dda_ctx_t ctx, void* dst, const void* src, size_t bytes);
+dda_status dda_memcpy_d2d(dda_ctx_t ctx, void* dst, const void* src, size_t bytes);
 
 /* FFT grid size for a box extent of x dipoles: fftFit, DDA/DDAParams.cpp:45-90.
  * x is 2*box; override != 0 returns override (0 if it is smaller than x). */
