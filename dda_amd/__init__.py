@@ -33,9 +33,11 @@ SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_
 EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
-    "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_apply",
+    "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_set_cc_sqrt_batch",
+    "dda_op_apply", "dda_op_apply_batch",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
-    "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_history",
+    "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
+    "dda_solver_history",
     "dda_solver_set_inner_floor", "dda_solver_profile_read",
 ]
 
@@ -61,10 +63,10 @@ class OpDesc(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint64), ("matvecs", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
                 ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 _lib = None
@@ -96,7 +98,9 @@ def lib():
         L.dda_op_destroy.argtypes = [vp]
         L.dda_op_info.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(sz)]
         L.dda_op_set_cc_sqrt.argtypes = [vp, dp]
+        L.dda_op_set_cc_sqrt_batch.argtypes = [vp, dp, i32]
         L.dda_op_apply.argtypes = [vp, vp, vp, i32, vp]
+        L.dda_op_apply_batch.argtypes = [vp, vp, vp, i32, i32, vp]
         L.dda_op_apply_host.argtypes = [vp, dp, dp, i32]
         L.dda_op_get_dmatrix.argtypes = [vp, dp]
         L.dda_op_profile.argtypes = [vp, i32]
@@ -107,6 +111,7 @@ def lib():
         L.dda_solver_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_destroy.argtypes = [vp]
         L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
+        L.dda_solver_solve_batch.argtypes = [vp, dp, i32, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_history.argtypes = [vp, dp, u64, ctypes.POINTER(u64)]
         for name in EXPORTS:
             f = getattr(L, name)
@@ -286,11 +291,25 @@ class GpuMatVec:
         cs = np.ascontiguousarray(np.asarray(cc_sqrt, np.complex128).reshape(self.nmat, 3))
         _check(lib().dda_op_set_cc_sqrt(self.handle, _dp(cs.view(np.float64))), "dda_op_set_cc_sqrt")
 
+    def set_couple_constants_batch(self, cc_sqrt):
+        """One set of couple constants per right-hand side: [nrhs, nmat, 3] complex."""
+        cs = np.asarray(cc_sqrt, np.complex128)
+        nrhs = cs.shape[0]
+        cs = np.ascontiguousarray(cs.reshape(nrhs, self.nmat, 3))
+        _check(lib().dda_op_set_cc_sqrt_batch(self.handle, _dp(cs.view(np.float64)), nrhs),
+               "dda_op_set_cc_sqrt_batch")
+
     def apply(self, x, y, conj=False, stream=None):
         """Device vectors (DeviceVector or raw pointers); enqueued, not synchronised."""
         xp = x.ptr if isinstance(x, DeviceVector) else x
         yp = y.ptr if isinstance(y, DeviceVector) else y
         _check(lib().dda_op_apply(self.handle, xp, yp, int(conj), stream), "dda_op_apply")
+
+    def apply_batch(self, x, y, nrhs, conj=False, stream=None):
+        """Block matvec over nrhs vectors stored back to back ([nrhs][3][vec_stride])."""
+        xp = x.ptr if isinstance(x, DeviceVector) else x
+        yp = y.ptr if isinstance(y, DeviceVector) else y
+        _check(lib().dda_op_apply_batch(self.handle, xp, yp, int(nrhs), int(conj), stream), "dda_op_apply_batch")
 
     def dmatrix(self):
         """The device tensor in the reference layout [Gx, Gz, Gy, 6] (1/m^3)."""
@@ -377,6 +396,27 @@ class IterativeSolver:
         return out
 
     getPolVec = get_pol_vec
+
+    def get_pol_vecs(self, einc, eps, start=None):
+        """Solves a batch of incident fields [nrhs, 3*vec_stride] in lockstep.
+
+        Returns (pol [nrhs, 3*vec_stride], infos: list of dicts with a per-RHS
+        status); raises only for errors of the call itself."""
+        e = np.ascontiguousarray(einc, np.complex128)
+        nrhs = e.shape[0]
+        assert e.shape[1] == self.op.vec_size
+        out = np.zeros_like(e)
+        st = None if start is None else np.ascontiguousarray(start, np.complex128)
+        infos = (SolveInfo * nrhs)()
+        status = lib().dda_solver_solve_batch(self.handle, _dp(e.view(np.float64)), nrhs, float(eps),
+                                              None if st is None else _dp(st.view(np.float64)),
+                                              _dp(out.view(np.float64)), infos)
+        self.infos = [i.as_dict() for i in infos]
+        self.info = self.infos[0]
+        if status not in (OK, BREAKDOWN, MAX_ITER, STAGNATION, INVALID_ARG) or (
+                status == INVALID_ARG and not any(i["status"] == INVALID_ARG for i in self.infos)):
+            _check(status, "get_pol_vecs")
+        return out, self.infos
 
     def profile_read(self):
         """Kernel times of the operator the iteration runs on (FP32 twin when mixed)."""

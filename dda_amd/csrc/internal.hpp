@@ -38,6 +38,16 @@ struct MvDims {
   int nv;
   int S;       // vector stride
   int nrows;   // By * Bz
+  int nmat;
+};
+
+// Per-right-hand-side stop flags of a device-side solver: the flag of RHS r is
+// done[r * stride]; null = never stop. Kernels whose blockIdx.y runs over
+// batches map batch b to RHS b / per.
+struct RhsFlags {
+  const int* done = nullptr;
+  int stride = 0;
+  int per = 1;
 };
 
 // Build a plan: radix-8 stages first, then 4, 2, 3, 5, 7. Returns false if n
@@ -48,9 +58,11 @@ bool make_plan(int n, FftPlan* p);
 dda_status make_twiddles(int n, int prec, void** out);
 
 // Launch helpers implemented in matvec.hip.
+// nrhs right-hand sides stored back to back ([nrhs][3][S]); done (nullable)
+// points at the first RHS's stop flag, done_stride ints apart.
 template <typename R>
 dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj, hipStream_t st,
-                         const int* done);
+                         const int* done, int nrhs = 1, int done_stride = 0);
 
 }  // namespace dda
 
@@ -97,5 +109,7 @@ struct dda_op_s {
   // launch shapes
   int p1_rows = 1;   // rows per work-group in the X passes
   int col_t = 8;     // columns per work-group in the Y/Z passes
+  int cap_rhs = 1;   // right-hand sides the grid buffers hold
+  int cc_rhs = 0;    // right-hand sides with couple constants set
   bool static_z = false;  // pass 3 uses a compile-time planned kernel
 };

@@ -1,4 +1,5 @@
-// Device-resident Krylov solvers: QMR-CS, BiCGStab, CGNR and BiCG-CS.
+// Device-resident Krylov solvers: QMR-CS, BiCGStab, CGNR and BiCG-CS, for one
+// or a batch of right-hand sides.
 //
 // Replaces IterativeSolverBase::getPolVec (DDA/IterativeSolverBase.cpp:40-103),
 // GpuIterativeSolver::initGeneral/getResult (DDA/GpuIterativeSolver.cpp:78-144),
@@ -8,14 +9,18 @@
 // the reference's CPU solvers line by line (DDA/QmrCs.cpp:78-168,
 // DDA/BicgStab.cpp:66-112, DDA/Cgnr.cpp:43-73, DDA/BicgCs.cpp:58-84).
 //
-// Every scalar lives in device memory (SolverState, FP64). Reductions are
-// two-stage with a fixed grid, so their summation order, and hence the result,
-// depends only on the vector length. Scalar algebra runs in a one-block kernel.
-// The loop test, the maximum-iteration and stagnation aborts
-// (IterativeSolverBase.cpp:56-70) and the breakdown tests are evaluated on the
-// device: once `done` is set every later kernel of the solve returns at entry,
-// so the host can enqueue iterations ahead and only reads the flag once per
-// chunk of iterations.
+// Every scalar lives in device memory (one SolverState per right-hand side,
+// FP64). Reductions are two-stage with a fixed number of blocks per
+// right-hand side that depends only on the vector length, so a right-hand
+// side's result is bitwise the same alone and in any batch. Scalar algebra runs
+// in a one-block-per-RHS kernel. The loop test, the maximum-iteration and
+// stagnation aborts (IterativeSolverBase.cpp:56-70) and the breakdown tests are
+// evaluated on the device: once a right-hand side's `done` is set every later
+// kernel skips it (its matvec included), so the host enqueues iterations ahead
+// and reads the states once per chunk of iterations.
+//
+// Vector kernels run on a grid (nb, nrhs): blockIdx.y is the right-hand side,
+// vectors are [nrhs][n] back to back.
 
 #include <chrono>
 #include <cmath>
@@ -25,9 +30,15 @@
 
 namespace dda {
 
-constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
-constexpr int kChunk = 16;  // most iterations enqueued between flag reads
+constexpr int kMaxRedBlocks = 1024;
+constexpr int kChunk = 16;  // most iterations enqueued between state reads
+
+// Reduction blocks per right-hand side for vectors of n complex values.
+inline int red_blocks(long long n) {
+  long long b = (n + 8 * kRedThreads - 1) / (8 * kRedThreads);
+  return (int)(b < 1 ? 1 : (b > kMaxRedBlocks ? kMaxRedBlocks : b));
+}
 
 struct SolverState {
   int done;     // 1 once the solve has finished for any reason
@@ -35,7 +46,7 @@ struct SolverState {
   int skip;     // done, or BiCGStab's early exit inside the current iteration
   int sel;      // initGeneral: 1 = start from x = 0
   unsigned long long count, counter, max_iter, max_res_increase, hist_cap;
-  double residScale, epsB, inprodR, inprodR_init, ip1, temp;
+  double eps, residScale, epsB, inprodR, inprodR_init, ip1, temp;
   // QMR-CS (DDA/QmrCs.hpp VarType)
   double omega_old, omega_new, c_old, c_new;
   double2 beta, mBeta, tautilda, s_old, s_new;
@@ -44,6 +55,8 @@ struct SolverState {
   double2 ro_new, ro_old, omega, bbeta, mbo, mOmega, one, mOne;
   double denumOmega, abs_ro_new, cg_ro_new, cg_ro_old;
 };
+constexpr int kStateInts = sizeof(SolverState) / sizeof(int);
+static_assert(sizeof(SolverState) % sizeof(int) == 0, "state stride");
 
 __device__ inline double2 dmul(double2 a, double2 b) { return cmul(a, b); }
 __device__ inline double2 ddiv(double2 a, double2 b) {
@@ -83,45 +96,59 @@ __device__ inline double block_sum(double v) {
   return s;
 }
 
-// Writes up to four per-block partial sums.
-__device__ inline void write_partials(double* part, int nq, double q0, double q1, double q2, double q3) {
-  double t0 = block_sum(q0);
-  double t1 = nq > 1 ? block_sum(q1) : 0.0;
-  double t2 = nq > 2 ? block_sum(q2) : 0.0;
-  double t3 = nq > 3 ? block_sum(q3) : 0.0;
+// Per-block partial sums of up to three quantities; quantity q of block b of
+// right-hand side r sits at part[(r * 4 + q) * nb + b].
+__device__ inline void write_partials(double* part, int nq, double q0, double q1, double q2) {
+  const int nb = gridDim.x;
+  double* p = part + (long long)blockIdx.y * 4 * nb;
+  const double t0 = block_sum(q0);
+  const double t1 = nq > 1 ? block_sum(q1) : 0.0;
+  const double t2 = nq > 2 ? block_sum(q2) : 0.0;
   if (threadIdx.x == 0) {
-    part[0 * kRedBlocks + blockIdx.x] = t0;
-    if (nq > 1) part[1 * kRedBlocks + blockIdx.x] = t1;
-    if (nq > 2) part[2 * kRedBlocks + blockIdx.x] = t2;
-    if (nq > 3) part[3 * kRedBlocks + blockIdx.x] = t3;
+    p[blockIdx.x] = t0;
+    if (nq > 1) p[nb + blockIdx.x] = t1;
+    if (nq > 2) p[2 * nb + blockIdx.x] = t2;
   }
 }
 
-// Second stage, run by a whole single block: fixed order.
-__device__ inline double sum_partials(const double* part, int q) {
+// Second stage, run by a whole block for one right-hand side: fixed order.
+__device__ inline double sum_partials(const double* part_rhs, int nb, int q) {
   double s = 0;
-  for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) s += part[q * kRedBlocks + i];
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part_rhs[q * nb + i];
   return block_sum(s);
 }
 
-template <typename C> __device__ inline C coef(const double2* p) {
-  return cmake<C>(p->x, p->y);
-}
+template <typename C> __device__ inline C coef(const double2* p) { return cmake<C>(p->x, p->y); }
+
+#define DDA_GRID_LOOP(i) \
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+
+// Per-RHS set-up of a vector kernel: the state and the vectors' offsets.
+#define DDA_RHS_PROLOGUE(stp) \
+  const int rhs_ = blockIdx.y; \
+  const long long off_ = (long long)rhs_ * n; \
+  (void)off_; \
+  const SolverState* s_ = (stp) + rhs_; \
+  (void)s_;
 
 // ------------------------------------------------------------------------------------
-// Vector kernels (grid kRedBlocks x kRedThreads, grid-stride)
+// Vector kernels
 // ------------------------------------------------------------------------------------
 
 enum RedOp { RED_DOTU = 0, RED_DOTC = 1, RED_NORM = 2 };
 
-// Partial sums of a.b (DOTU), a.conj(b) (DOTC) or |a|^2 (NORM).
+// Partial sums of a.b (DOTU), a.conj(b) (DOTC) or |a|^2 (NORM). flag_off selects
+// the state field that masks the kernel (offset of done or skip in ints).
 template <typename R, int OP>
 __global__ __launch_bounds__(kRedThreads) void k_reduce(const typename CxT<R>::T* __restrict__ a,
                                                         const typename CxT<R>::T* __restrict__ b, long long n,
-                                                        double* part, const int* flag) {
-  if (*flag) return;
+                                                        double* part, const SolverState* st, int flag_off) {
+  DDA_RHS_PROLOGUE(st)
+  if (reinterpret_cast<const int*>(s_)[flag_off]) return;
+  a += off_;
+  b += off_;
   double s0 = 0, s1 = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     const double ar = a[i].x, ai = a[i].y;
     if (OP == RED_NORM) {
       s0 += ar * ar + ai * ai;
@@ -131,30 +158,36 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const typename CxT<R>::T
       s1 += ar * bi + ai * br;
     }
   }
-  write_partials(part, OP == RED_NORM ? 1 : 2, s0, s1, 0, 0);
+  write_partials(part, OP == RED_NORM ? 1 : 2, s0, s1, 0);
 }
 
-// out = c1 v1 + c2 v2 + v3 (c = nullptr: the term is absent; v3 may be null),
-// with an optional |out|^2 partial (quantity 0).
+// out = c1 v1 + c2 v2 + v3, coefficients are state fields (byte offsets, -1 =
+// term absent; v3 may be null), optional |out|^2 partial.
 template <typename R>
-__global__ __launch_bounds__(kRedThreads) void k_lincomb(typename CxT<R>::T* out, const typename CxT<R>::T* v1,
-                                                         const double2* c1, const typename CxT<R>::T* v2,
-                                                         const double2* c2, const typename CxT<R>::T* v3,
-                                                         long long n, double* part, const int* flag) {
+__global__ __launch_bounds__(kRedThreads) void k_lincomb(typename CxT<R>::T* out, const typename CxT<R>::T* v1, int c1,
+                                                         const typename CxT<R>::T* v2, int c2,
+                                                         const typename CxT<R>::T* v3, long long n, double* part,
+                                                         const SolverState* st, int flag_off) {
   typedef typename CxT<R>::T C;
-  if (*flag) return;
-  const C k1 = c1 ? coef<C>(c1) : czero<C>();
-  const C k2 = c2 ? coef<C>(c2) : czero<C>();
+  DDA_RHS_PROLOGUE(st)
+  if (reinterpret_cast<const int*>(s_)[flag_off]) return;
+  const char* sb = reinterpret_cast<const char*>(s_);
+  const C k1 = c1 >= 0 ? coef<C>(reinterpret_cast<const double2*>(sb + c1)) : czero<C>();
+  const C k2 = c2 >= 0 ? coef<C>(reinterpret_cast<const double2*>(sb + c2)) : czero<C>();
+  out += off_;
+  if (v1) v1 += off_;
+  if (v2) v2 += off_;
+  if (v3) v3 += off_;
   double s = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     C o = czero<C>();
-    if (c1) o = cmul(v1[i], k1);
-    if (c2) o = cadd(o, cmul(v2[i], k2));
+    if (c1 >= 0) o = cmul(v1[i], k1);
+    if (c2 >= 0) o = cadd(o, cmul(v2[i], k2));
     if (v3) o = cadd(o, v3[i]);
     out[i] = o;
     s += (double)o.x * o.x + (double)o.y * o.y;
   }
-  if (part) write_partials(part, 1, s, 0, 0, 0);
+  if (part) write_partials(part, 1, s, 0, 0);
 }
 
 // QMR-CS: vtilda = [vtilda * mBeta] + v * mAlpha + Av, with partials of
@@ -164,10 +197,14 @@ __global__ __launch_bounds__(kRedThreads) void k_qmr_vt(typename CxT<R>::T* vt, 
                                                         const typename CxT<R>::T* __restrict__ av, int first,
                                                         const SolverState* st, long long n, double* part) {
   typedef typename CxT<R>::T C;
-  if (st->done) return;
-  const C mb = coef<C>(&st->mBeta), ma = coef<C>(&st->mAlpha);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  vt += off_;
+  v += off_;
+  av += off_;
+  const C mb = coef<C>(&s_->mBeta), ma = coef<C>(&s_->mAlpha);
   double s0 = 0, s1 = 0, s2 = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     C o = cadd(cmul(v[i], ma), av[i]);
     if (!first) o = cadd(cmul(vt[i], mb), o);
     vt[i] = o;
@@ -176,12 +213,12 @@ __global__ __launch_bounds__(kRedThreads) void k_qmr_vt(typename CxT<R>::T* vt, 
     s1 += 2 * r * im;
     s2 += r * r + im * im;
   }
-  write_partials(part, 3, s0, s1, s2, 0);
+  write_partials(part, 3, s0, s1, s2);
 }
 
 // QMR-CS update, QmrCs.cpp:150-166: p_dst = [p_old mThetaZeta] + [p_new mEtaZeta]
 // + v zetaInv; x += p_dst tau; vt *= betaInv; r = r normSNew + vt cnot.
-// mode 0: nr == 0, 1: nr == 1, 2: later. Partial |r|^2 (quantity 0).
+// mode 0: nr == 0, 1: nr == 1, 2: later. Partial |r|^2.
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_qmr_upd(typename CxT<R>::T* p_dst, const typename CxT<R>::T* p_new,
                                                          const typename CxT<R>::T* __restrict__ v,
@@ -189,11 +226,18 @@ __global__ __launch_bounds__(kRedThreads) void k_qmr_upd(typename CxT<R>::T* p_d
                                                          typename CxT<R>::T* r, int mode, const SolverState* st,
                                                          long long n, double* part) {
   typedef typename CxT<R>::T C;
-  if (st->done) return;
-  const C zi = coef<C>(&st->zetaInv), mez = coef<C>(&st->mEtaZeta), mtz = coef<C>(&st->mThetaZeta);
-  const C tau = coef<C>(&st->tau), bi = coef<C>(&st->betaInv), cn = coef<C>(&st->cnot), ns = coef<C>(&st->normSNew);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  p_dst += off_;
+  p_new += off_;
+  v += off_;
+  vt += off_;
+  x += off_;
+  r += off_;
+  const C zi = coef<C>(&s_->zetaInv), mez = coef<C>(&s_->mEtaZeta), mtz = coef<C>(&s_->mThetaZeta);
+  const C tau = coef<C>(&s_->tau), bi = coef<C>(&s_->betaInv), cn = coef<C>(&s_->cnot), ns = coef<C>(&s_->normSNew);
   double s = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     C p = cmul(v[i], zi);
     if (mode >= 1) p = cadd(cmul(p_new[i], mez), p);
     if (mode >= 2) p = cadd(cmul(p_dst[i], mtz), p);
@@ -205,35 +249,41 @@ __global__ __launch_bounds__(kRedThreads) void k_qmr_upd(typename CxT<R>::T* p_d
     r[i] = rn;
     s += (double)rn.x * rn.x + (double)rn.y * rn.y;
   }
-  write_partials(part, 1, s, 0, 0, 0);
+  write_partials(part, 1, s, 0, 0);
 }
 
-// b = sqrt(a) einc (GpuIterativeSolverInst.cl setupPvec), partial |b|^2.
+// b = sqrt(a) einc (GpuIterativeSolverInst.cl setupPvec) with each RHS's couple
+// constants; einc is FP64 [nrhs][3][S].
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_setup_b(typename CxT<R>::T* b, const double2* __restrict__ einc,
                                                          const typename CxT<R>::T* __restrict__ ccs,
-                                                         const uint8_t* __restrict__ mat, int nv, int S, double* part) {
+                                                         const uint8_t* __restrict__ mat, int nv, int S, int nmat) {
   typedef typename CxT<R>::T C;
-  double s = 0;
   const long long n = 3ll * S;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+  const int rhs = blockIdx.y;
+  b += rhs * n;
+  einc += rhs * n;
+  ccs += rhs * 3 * nmat;
+  DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     C o = czero<C>();
     if (i < nv) o = cmul(ccs[3 * mat[i] + c], cmake<C>(einc[t].x, einc[t].y));
     b[t] = o;
-    s += (double)o.x * o.x + (double)o.y * o.y;
   }
-  write_partials(part, 1, s, 0, 0, 0);
 }
 
 // x = start / sqrt(a) (setupXvecStartValue), start given as p in d^3 units.
 template <typename R>
 __global__ void k_setup_start(typename CxT<R>::T* x, const double2* __restrict__ start,
                               const typename CxT<R>::T* __restrict__ ccs, const uint8_t* __restrict__ mat, int nv,
-                              int S, double scale) {
+                              int S, int nmat, double scale) {
   typedef typename CxT<R>::T C;
   const long long n = 3ll * S;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+  const int rhs = blockIdx.y;
+  x += rhs * n;
+  start += rhs * n;
+  ccs += rhs * 3 * nmat;
+  DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     double2 o = make_double2(0, 0);
     if (i < nv) {
@@ -248,10 +298,14 @@ __global__ void k_setup_start(typename CxT<R>::T* x, const double2* __restrict__
 template <typename R>
 __global__ void k_get_result(double2* out, const typename CxT<R>::T* __restrict__ x,
                              const typename CxT<R>::T* __restrict__ ccs, const uint8_t* __restrict__ mat, int nv, int S,
-                             double scale) {
+                             int nmat, double scale) {
   typedef typename CxT<R>::T C;
   const long long n = 3ll * S;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+  const int rhs = blockIdx.y;
+  out += rhs * n;
+  x += rhs * n;
+  ccs += rhs * 3 * nmat;
+  DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     double2 o = make_double2(0, 0);
     if (i < nv) {
@@ -269,8 +323,13 @@ template <typename R>
 __global__ void k_select_x0(typename CxT<R>::T* x, typename CxT<R>::T* r, const typename CxT<R>::T* __restrict__ b,
                             long long n, const SolverState* st) {
   typedef typename CxT<R>::T C;
-  const int sel = st->sel;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  x += off_;
+  r += off_;
+  b += off_;
+  const int sel = s_->sel;
+  DDA_GRID_LOOP(i) {
     if (sel) {
       x[i] = czero<C>();
       r[i] = b[i];
@@ -282,14 +341,16 @@ __global__ void k_select_x0(typename CxT<R>::T* x, typename CxT<R>::T* r, const 
 
 template <typename R>
 __global__ void k_copy_flag(typename CxT<R>::T* dst, const typename CxT<R>::T* __restrict__ src, long long n,
-                            const int* flag) {
-  if (*flag) return;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+                            const SolverState* st) {
+  DDA_RHS_PROLOGUE(st)
+  if (s_->skip) return;
+  dst += off_;
+  src += off_;
+  DDA_GRID_LOOP(i) dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------------------------
-// Scalar kernels (one block of kRedThreads)
+// Scalar kernel: one block per right-hand side
 // ------------------------------------------------------------------------------------
 
 enum Step {
@@ -344,9 +405,13 @@ __device__ inline void loop_head(SolverState* st, int kind) {
   st->skip = st->done;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st, const double* part, int step, int kind,
-                                                        double eps, double* hist) {
-  if (st->done && step >= S_LOOP_CHECK) return;
+__global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, const double* part_all, int nb, int step,
+                                                        int kind, double* hist_all) {
+  const int rhs = blockIdx.x;
+  SolverState* st = st_all + rhs;
+  const double* part = part_all + (long long)rhs * 4 * nb;
+  double* hist = hist_all ? hist_all + (long long)rhs * st->hist_cap : nullptr;
+  if (st->done) return;  // finished, or masked out of this solve
   // All threads take part in the reductions; thread 0 does the algebra.
   double q0 = 0, q1 = 0, q2 = 0;
   switch (step) {
@@ -356,7 +421,8 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st, const d
     case S_END_ITER:
     case S_CG_RO:
     case S_CG_ALPHA:
-      q0 = sum_partials(part, 0);
+    case S_BS_S:
+      q0 = sum_partials(part, nb, 0);
       break;
     case S_QMR_INIT:
     case S_QMR_ALPHA:
@@ -364,21 +430,14 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st, const d
     case S_BS_ALPHA:
     case S_BI_RO:
     case S_BI_MU:
-      q0 = sum_partials(part, 0);
-      q1 = sum_partials(part, 1);
+      q0 = sum_partials(part, nb, 0);
+      q1 = sum_partials(part, nb, 1);
       break;
     case S_QMR_GIVENS:
-      q0 = sum_partials(part, 0);
-      q1 = sum_partials(part, 1);
-      q2 = sum_partials(part, 2);
-      break;
-    case S_BS_S:
-      q0 = sum_partials(part, 0);
-      break;
     case S_BS_OMEGA:
-      q0 = sum_partials(part, 0);  // |Av|^2
-      q1 = sum_partials(part, 1);  // s.conj(Av)
-      q2 = sum_partials(part, 2);
+      q0 = sum_partials(part, nb, 0);
+      q1 = sum_partials(part, nb, 1);
+      q2 = sum_partials(part, nb, 2);
       break;
     default:
       break;
@@ -392,7 +451,7 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st, const d
         break;
       }
       st->residScale = 1.0 / q0;
-      st->epsB = eps * eps / st->residScale;
+      st->epsB = st->eps * st->eps / st->residScale;
       st->one = dre(1.0);
       st->mOne = dre(-1.0);
       break;
@@ -563,10 +622,12 @@ template <typename R>
 __global__ void k_bs_early(typename CxT<R>::T* x, const typename CxT<R>::T* __restrict__ p, long long n,
                            const SolverState* st) {
   typedef typename CxT<R>::T C;
-  if (st->done || !st->skip) return;
-  const C a = coef<C>(&st->alpha);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    x[i] = cadd(cmul(p[i], a), x[i]);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done || !s_->skip) return;
+  x += off_;
+  p += off_;
+  const C a = coef<C>(&s_->alpha);
+  DDA_GRID_LOOP(i) x[i] = cadd(cmul(p[i], a), x[i]);
 }
 
 // BiCGStab: p = p beta + v (-beta omega) + r (nr > 0) or p = r.
@@ -575,9 +636,13 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_p(typename CxT<R>::T* p, con
                                                       const typename CxT<R>::T* __restrict__ r, int first,
                                                       const SolverState* st, long long n) {
   typedef typename CxT<R>::T C;
-  if (st->done) return;
-  const C b = coef<C>(&st->bbeta), mbo = coef<C>(&st->mbo);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  p += off_;
+  v += off_;
+  r += off_;
+  const C b = coef<C>(&s_->bbeta), mbo = coef<C>(&s_->mbo);
+  DDA_GRID_LOOP(i) {
     if (first)
       p[i] = r[i];
     else
@@ -590,15 +655,18 @@ template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_bs_om(const typename CxT<R>::T* __restrict__ av,
                                                        const typename CxT<R>::T* __restrict__ s, long long n,
                                                        double* part, const SolverState* st) {
-  if (st->skip) return;
+  DDA_RHS_PROLOGUE(st)
+  if (s_->skip) return;
+  av += off_;
+  s += off_;
   double n2 = 0, dr = 0, di = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     const double ar = av[i].x, ai = av[i].y, sr = s[i].x, si = s[i].y;
     n2 += ar * ar + ai * ai;
     dr += sr * ar + si * ai;
     di += si * ar - sr * ai;
   }
-  write_partials(part, 3, n2, dr, di, 0);
+  write_partials(part, 3, n2, dr, di);
 }
 
 // BiCGStab: x = p alpha + s omega + x; r = Av (-omega) + s; partial |r|^2.
@@ -607,24 +675,31 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, t
                                                         const typename CxT<R>::T* __restrict__ p,
                                                         const typename CxT<R>::T* __restrict__ s,
                                                         const typename CxT<R>::T* __restrict__ av, long long n,
-                                                        double* part, SolverState* st) {
+                                                        double* part, const SolverState* st) {
   typedef typename CxT<R>::T C;
-  if (st->skip) return;
-  const C a = coef<C>(&st->alpha), om = coef<C>(&st->omega), mom = coef<C>(&st->mOmega);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->skip) return;
+  x += off_;
+  r += off_;
+  p += off_;
+  s += off_;
+  av += off_;
+  const C a = coef<C>(&s_->alpha), om = coef<C>(&s_->omega), mom = coef<C>(&s_->mOmega);
   double sum = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     x[i] = cadd(cadd(cmul(p[i], a), cmul(s[i], om)), x[i]);
     const C rn = cadd(cmul(av[i], mom), s[i]);
     r[i] = rn;
     sum += (double)rn.x * rn.x + (double)rn.y * rn.y;
   }
-  write_partials(part, 1, sum, 0, 0, 0);
+  write_partials(part, 1, sum, 0, 0);
 }
 
 // BiCGStab end of iteration: ip1 from |r|^2 unless the early exit was taken.
-__global__ __launch_bounds__(kRedThreads) void k_bs_fin(SolverState* st, const double* part) {
+__global__ __launch_bounds__(kRedThreads) void k_bs_fin(SolverState* st_all, const double* part_all, int nb) {
+  SolverState* st = st_all + blockIdx.x;
   if (st->done) return;
-  const double q0 = sum_partials(part, 0);
+  const double q0 = sum_partials(part_all + (long long)blockIdx.x * 4 * nb, nb, 0);
   if (threadIdx.x == 0 && !st->skip) {
     st->ip1 = q0;
     st->ro_old = st->ro_new;
@@ -636,10 +711,12 @@ template <typename R>
 __global__ void k_p_update(typename CxT<R>::T* p, const typename CxT<R>::T* __restrict__ a, int first,
                            const SolverState* st, long long n) {
   typedef typename CxT<R>::T C;
-  if (st->done) return;
-  const C b = coef<C>(&st->bbeta);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    p[i] = first ? a[i] : cadd(cmul(p[i], b), a[i]);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  p += off_;
+  a += off_;
+  const C b = coef<C>(&s_->bbeta);
+  DDA_GRID_LOOP(i) p[i] = first ? a[i] : cadd(cmul(p[i], b), a[i]);
 }
 
 // x = p alpha + x; r = Av mAlpha + r; partial |r|^2 (CGNR, BiCG-CS).
@@ -649,44 +726,84 @@ __global__ __launch_bounds__(kRedThreads) void k_xr_update(typename CxT<R>::T* x
                                                            const typename CxT<R>::T* __restrict__ av, long long n,
                                                            double* part, const SolverState* st) {
   typedef typename CxT<R>::T C;
-  if (st->done) return;
-  const C a = coef<C>(&st->alpha), ma = coef<C>(&st->mAlpha);
+  DDA_RHS_PROLOGUE(st)
+  if (s_->done) return;
+  x += off_;
+  r += off_;
+  p += off_;
+  av += off_;
+  const C a = coef<C>(&s_->alpha), ma = coef<C>(&s_->mAlpha);
   double sum = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  DDA_GRID_LOOP(i) {
     x[i] = cadd(cmul(p[i], a), x[i]);
     const C rn = cadd(cmul(av[i], ma), r[i]);
     r[i] = rn;
     sum += (double)rn.x * rn.x + (double)rn.y * rn.y;
   }
-  write_partials(part, 1, sum, 0, 0, 0);
+  write_partials(part, 1, sum, 0, 0);
 }
 
-// Mixed precision helpers: r32 = (float)(r64 * scale); x64 += (double)d32 * scale;
-// r64 = b64 - t64 with partial |r|^2.
-__global__ void k_to_f32(float2* out, const double2* __restrict__ in, long long n, double scale) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = make_float2((float)(in[i].x * scale), (float)(in[i].y * scale));
+// Mixed precision helpers, per right-hand side (blockIdx.y): out = r64 * scale[r]
+// in the inner solver's precision; x64 += d * scale[r]; r64 = b64 - t64 with
+// partials of |r|^2 and r^T r (quantities 0; 1, 2). mask[r] != 0 skips a
+// right-hand side.
+template <typename C>
+__global__ void k_load_rhs(C* out, const double2* __restrict__ in, long long n, const double* scale, const int* mask) {
+  if (mask[blockIdx.y]) return;
+  const double sc = scale[blockIdx.y];
+  out += blockIdx.y * n;
+  in += blockIdx.y * n;
+  DDA_GRID_LOOP(i) out[i] = cmake<C>(in[i].x * sc, in[i].y * sc);
 }
-__global__ void k_add_f32(double2* x, const float2* __restrict__ d, long long n, double scale) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    x[i].x += (double)d[i].x * scale;
-    x[i].y += (double)d[i].y * scale;
+template <typename C>
+__global__ void k_add_corr(double2* x, const C* __restrict__ d, long long n, const double* scale, const int* mask) {
+  if (mask[blockIdx.y]) return;
+  const double sc = scale[blockIdx.y];
+  x += blockIdx.y * n;
+  d += blockIdx.y * n;
+  DDA_GRID_LOOP(i) {
+    x[i].x += (double)d[i].x * sc;
+    x[i].y += (double)d[i].y * sc;
   }
 }
 __global__ __launch_bounds__(kRedThreads) void k_resid64(double2* r, const double2* __restrict__ b,
-                                                         const double2* __restrict__ t, long long n, double* part) {
-  double s = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+                                                         const double2* __restrict__ t, long long n, double* part,
+                                                         const int* mask) {
+  if (mask[blockIdx.y]) return;
+  r += blockIdx.y * n;
+  b += blockIdx.y * n;
+  t += blockIdx.y * n;
+  double s = 0, tr = 0, ti = 0;
+  DDA_GRID_LOOP(i) {
     const double2 v = make_double2(b[i].x - t[i].x, b[i].y - t[i].y);
     r[i] = v;
     s += v.x * v.x + v.y * v.y;
+    tr += v.x * v.x - v.y * v.y;
+    ti += 2 * v.x * v.y;
   }
-  write_partials(part, 1, s, 0, 0, 0);
+  write_partials(part, 3, s, tr, ti);
+}
+__global__ __launch_bounds__(kRedThreads) void k_norm64(const double2* __restrict__ a, long long n, double* part) {
+  a += blockIdx.y * n;
+  double s = 0, tr = 0, ti = 0;
+  DDA_GRID_LOOP(i) {
+    s += a[i].x * a[i].x + a[i].y * a[i].y;
+    tr += a[i].x * a[i].x - a[i].y * a[i].y;
+    ti += 2 * a[i].x * a[i].y;
+  }
+  write_partials(part, 3, s, tr, ti);
 }
 
-// One Krylov solve A x = b on the device for an operator of precision R.
-// v[0] = b (set by the caller), v[2] = x (x0 when have_x0), v[1] = r, v[3] = Av,
-// v[4..7] solver work vectors.
+// ------------------------------------------------------------------------------------
+// Host side: one batched Krylov core per precision
+// ------------------------------------------------------------------------------------
+
+#define DDA_OFF(field) ((int)(offsetof(SolverState, field)))
+#define DDA_IOFF(field) ((int)(offsetof(SolverState, field) / sizeof(int)))
+
+// One Krylov solve A x = b for up to `cap` right-hand sides on the device for an
+// operator of precision R. v[0] = b (set by the caller), v[2] = x (x0 when
+// have_x0), v[1] = r, v[3] = Av, v[4..7] solver work vectors; each [cap][n].
 template <typename R>
 struct Core {
   typedef typename CxT<R>::T C;
@@ -694,48 +811,54 @@ struct Core {
   int kind = 0;
   hipStream_t st = nullptr;
   long long n = 0;
-  SolverState* dst = nullptr;       // device
-  SolverState* host2 = nullptr;     // pinned, two slots
-  double* part = nullptr;
-  double* hist = nullptr;
+  int nb = 1;                       // reduction blocks per right-hand side
+  int cap = 0;                      // right-hand sides allocated
+  int nrhs = 1;                     // right-hand sides of the current solve
+  SolverState* dst = nullptr;       // device [cap]
+  SolverState* host2 = nullptr;     // pinned [2][cap]
+  double* part = nullptr;           // [cap][4][nb]
+  double* hist = nullptr;           // [cap][hist_cap]
   uint64_t hist_cap = 0;
   uint64_t max_iter = 0, max_res_increase = 0;
   C* v[8] = {};
 
-  dim3 g() const { return dim3(kRedBlocks); }
+  dim3 g() const { return dim3(nb, nrhs); }
   dim3 b() const { return dim3(kRedThreads); }
-  dda_status matvec(const void* x, void* y, int conj, const int* flag) {
-    return launch_matvec<R>(op, x, y, conj, st, flag);
+  const int* flag(int ioff) const { return reinterpret_cast<const int*>(dst) + ioff; }
+  dda_status matvec(const void* x, void* y, int conj, int ioff) {
+    return launch_matvec<R>(op, x, y, conj, st, flag(ioff), nrhs, kStateInts);
   }
-  void scalar(int step, double eps = 0) {
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kRedThreads), 0, st, dst, part, step, kind, eps, hist);
+  void scalar(int step) {
+    hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kRedThreads), 0, st, dst, part, nb, step, kind, hist);
   }
-  void reduce(int opk, const C* a, const C* bb, const int* flag) {
+  void reduce(int opk, const C* a, const C* bb, int ioff) {
     if (opk == RED_NORM)
-      hipLaunchKernelGGL((k_reduce<R, RED_NORM>), g(), b(), 0, st, a, bb, n, part, flag);
+      hipLaunchKernelGGL((k_reduce<R, RED_NORM>), g(), b(), 0, st, a, bb, n, part, dst, ioff);
     else if (opk == RED_DOTU)
-      hipLaunchKernelGGL((k_reduce<R, RED_DOTU>), g(), b(), 0, st, a, bb, n, part, flag);
+      hipLaunchKernelGGL((k_reduce<R, RED_DOTU>), g(), b(), 0, st, a, bb, n, part, dst, ioff);
     else
-      hipLaunchKernelGGL((k_reduce<R, RED_DOTC>), g(), b(), 0, st, a, bb, n, part, flag);
+      hipLaunchKernelGGL((k_reduce<R, RED_DOTC>), g(), b(), 0, st, a, bb, n, part, dst, ioff);
   }
-  void lincomb(C* out, const C* v1, const double2* c1, const C* v2, const double2* c2, const C* v3, bool withpart,
-               const int* flag) {
-    hipLaunchKernelGGL(k_lincomb<R>, g(), b(), 0, st, out, v1, c1, v2, c2, v3, n, withpart ? part : nullptr, flag);
+  void lincomb(C* out, const C* v1, int c1, const C* v2, int c2, const C* v3, bool withpart, int ioff) {
+    hipLaunchKernelGGL(k_lincomb<R>, g(), b(), 0, st, out, v1, c1, v2, c2, v3, n, withpart ? part : nullptr, dst, ioff);
   }
 
-  dda_status alloc(dda_op_s* o, int k, uint64_t mi, uint64_t mri) {
+  dda_status alloc(dda_op_s* o, int k, uint64_t mi, uint64_t mri, int capacity) {
+    release();
     op = o;
     kind = k;
     st = o->ctx->stream;
     n = 3ll * o->dims.S;
+    nb = red_blocks(n);
+    cap = capacity;
     max_iter = mi;
     max_res_increase = mri;
-    hist_cap = std::min<uint64_t>(mi + 1, 1u << 20);
-    DDA_TRY_HIP(hipMalloc(&dst, sizeof(SolverState)));
-    DDA_TRY_HIP(hipHostMalloc((void**)&host2, 2 * sizeof(SolverState), hipHostMallocDefault));
-    DDA_TRY_HIP(hipMalloc(&part, 4 * kRedBlocks * sizeof(double)));
-    DDA_TRY_HIP(hipMalloc(&hist, hist_cap * sizeof(double)));
-    for (int i = 0; i < 8; i++) DDA_TRY_HIP(hipMalloc(&v[i], n * sizeof(C)));
+    hist_cap = std::min<uint64_t>(mi + 1, 1u << 16);
+    DDA_TRY_HIP(hipMalloc(&dst, cap * sizeof(SolverState)));
+    DDA_TRY_HIP(hipHostMalloc((void**)&host2, 2 * cap * sizeof(SolverState), hipHostMallocDefault));
+    DDA_TRY_HIP(hipMalloc(&part, (size_t)cap * 4 * nb * sizeof(double)));
+    DDA_TRY_HIP(hipMalloc(&hist, (size_t)cap * hist_cap * sizeof(double)));
+    for (int i = 0; i < 8; i++) DDA_TRY_HIP(hipMalloc(&v[i], (size_t)cap * n * sizeof(C)));
     return DDA_OK;
   }
   void release() {
@@ -750,52 +873,53 @@ struct Core {
     part = nullptr;
     hist = nullptr;
     for (auto& p : v) p = nullptr;
+    cap = 0;
   }
 
-  // Sum of partial quantity 0 on the host (synchronises).
-  dda_status read_sum(double* out) {
-    std::vector<double> h(kRedBlocks);
-    DDA_TRY_HIP(hipMemcpyAsync(h.data(), part, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
-    DDA_TRY_HIP(hipStreamSynchronize(st));
-    double t = 0;
-    for (double x : h) t += x;
-    *out = t;
-    return DDA_OK;
-  }
-
-  // initGeneral + getPolVec loop (GpuIterativeSolver.cpp:78-134,
-  // IterativeSolverBase.cpp:40-103); leaves x in v[2] and the final state in *fin.
-  dda_status solve(double eps, bool have_x0, SolverState* fin) {
-    const int* done = &dst->done;
-    const int* skip = &dst->skip;
+  // initGeneral + the getPolVec loop (GpuIterativeSolver.cpp:78-134,
+  // IterativeSolverBase.cpp:40-103) for right-hand sides 0..k-1 with eps[r];
+  // active[r] == 0 leaves r untouched (done from the start). Leaves x in v[2]
+  // and the final states in fin[0..k-1].
+  dda_status solve(int k, const double* eps, const int* active, bool have_x0, SolverState* fin,
+                   const uint64_t* max_iters = nullptr) {
+    nrhs = k;
     C* bv = v[0];
     C* r = v[1];
     C* x = v[2];
     C* av = v[3];
-    SolverState h;
-    std::memset(&h, 0, sizeof h);
-    h.max_iter = max_iter;
-    h.max_res_increase = max_res_increase;
-    h.hist_cap = hist_cap;
-    DDA_TRY_HIP(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, st));
-    reduce(RED_NORM, bv, bv, skip);
-    scalar(S_INIT_TEMP, eps);
+    const int DONE = DDA_IOFF(done), SKIP = DDA_IOFF(skip);
+    std::vector<SolverState> h(k);
+    for (int i = 0; i < k; i++) {
+      std::memset(&h[i], 0, sizeof(SolverState));
+      h[i].max_iter = max_iters ? max_iters[i] : max_iter;
+      h[i].max_res_increase = max_res_increase;
+      h[i].hist_cap = hist_cap;
+      h[i].eps = eps[i];
+      if (active && !active[i]) {
+        h[i].done = 1;
+        h[i].skip = 1;
+        h[i].status = DDA_OK;
+      }
+    }
+    DDA_TRY_HIP(hipMemcpyAsync(dst, h.data(), k * sizeof(SolverState), hipMemcpyHostToDevice, st));
+    reduce(RED_NORM, bv, bv, SKIP);
+    scalar(S_INIT_TEMP);
     if (have_x0) {
-      DDA_TRY(matvec(x, av, 0, nullptr));
-      lincomb(r, av, &dst->mOne, nullptr, nullptr, bv, true, skip);  // r = b - A x0
+      DDA_TRY(matvec(x, av, 0, DONE));
+      lincomb(r, av, DDA_OFF(mOne), nullptr, -1, bv, true, SKIP);  // r = b - A x0
       scalar(S_INIT_START);
     } else {
-      DDA_TRY(matvec(bv, av, 0, nullptr));
-      lincomb(r, av, &dst->mOne, nullptr, nullptr, bv, true, skip);  // r = b - A b
+      DDA_TRY(matvec(bv, av, 0, DONE));
+      lincomb(r, av, DDA_OFF(mOne), nullptr, -1, bv, true, SKIP);  // r = b - A b
       scalar(S_INIT_R);
       hipLaunchKernelGGL(k_select_x0<R>, g(), b(), 0, st, x, r, bv, n, dst);
     }
     if (kind == DDA_SOLVER_QMR_CS) {
-      reduce(RED_DOTU, r, r, skip);
+      reduce(RED_DOTU, r, r, SKIP);
       scalar(S_QMR_INIT);
-      lincomb(v[4], r, &dst->betaInv, nullptr, nullptr, nullptr, false, skip);  // v = r / beta
+      lincomb(v[4], r, DDA_OFF(betaInv), nullptr, -1, nullptr, false, SKIP);  // v = r / beta
     } else if (kind == DDA_SOLVER_BICGSTAB) {
-      hipLaunchKernelGGL(k_copy_flag<R>, g(), b(), 0, st, v[7], r, n, skip);  // rtilda = r
+      hipLaunchKernelGGL(k_copy_flag<R>, g(), b(), 0, st, v[7], r, n, dst);  // rtilda = r
     }
     scalar(S_LOOP_CHECK);
     DDA_TRY_HIP(hipGetLastError());
@@ -807,8 +931,8 @@ struct Core {
         case DDA_SOLVER_QMR_CS: {
           C* vv = v[iv];
           C* vt = v[ivt];
-          DDA_TRY(matvec(vv, av, 0, done));
-          reduce(RED_DOTU, vv, av, done);
+          DDA_TRY(matvec(vv, av, 0, DONE));
+          reduce(RED_DOTU, vv, av, DONE);
           scalar(S_QMR_ALPHA);
           hipLaunchKernelGGL(k_qmr_vt<R>, g(), b(), 0, st, vt, vv, av, nr == 0 ? 1 : 0, dst, n, part);
           scalar(S_QMR_GIVENS);
@@ -824,31 +948,31 @@ struct Core {
           C* vv = v[5];
           C* sv = v[6];
           C* rt = v[7];
-          reduce(RED_DOTC, r, rt, done);
+          reduce(RED_DOTC, r, rt, DONE);
           scalar(S_BS_RO);
           hipLaunchKernelGGL(k_bs_p<R>, g(), b(), 0, st, p, vv, r, nr == 0 ? 1 : 0, dst, n);
-          DDA_TRY(matvec(p, vv, 0, done));
-          reduce(RED_DOTC, vv, rt, done);
+          DDA_TRY(matvec(p, vv, 0, DONE));
+          reduce(RED_DOTC, vv, rt, DONE);
           scalar(S_BS_ALPHA);
-          lincomb(sv, vv, &dst->mAlpha, nullptr, nullptr, r, true, done);  // s = v (-alpha) + r
+          lincomb(sv, vv, DDA_OFF(mAlpha), nullptr, -1, r, true, DONE);  // s = v (-alpha) + r
           scalar(S_BS_S);
           hipLaunchKernelGGL(k_bs_early<R>, g(), b(), 0, st, x, p, n, dst);
-          DDA_TRY(matvec(sv, av, 0, skip));
+          DDA_TRY(matvec(sv, av, 0, SKIP));
           hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, av, sv, n, part, dst);
           scalar(S_BS_OMEGA);
           hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, n, part, dst);
-          hipLaunchKernelGGL(k_bs_fin, dim3(1), b(), 0, st, dst, part);
+          hipLaunchKernelGGL(k_bs_fin, dim3(nrhs), b(), 0, st, dst, part, nb);
           scalar(S_END_ITER);
           break;
         }
         case DDA_SOLVER_CGNR: {
           C* p = v[4];
-          DDA_TRY(matvec(r, av, 1, done));  // A^H r
-          reduce(RED_NORM, av, av, done);
+          DDA_TRY(matvec(r, av, 1, DONE));  // A^H r
+          reduce(RED_NORM, av, av, DONE);
           scalar(S_CG_RO);
           hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, av, nr == 0 ? 1 : 0, dst, n);
-          DDA_TRY(matvec(p, av, 0, done));
-          reduce(RED_NORM, av, av, done);
+          DDA_TRY(matvec(p, av, 0, DONE));
+          reduce(RED_NORM, av, av, DONE);
           scalar(S_CG_ALPHA);
           hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
           scalar(S_END_ITER);
@@ -856,11 +980,11 @@ struct Core {
         }
         case DDA_SOLVER_BICG_CS: {
           C* p = v[4];
-          reduce(RED_DOTU, r, r, done);
+          reduce(RED_DOTU, r, r, DONE);
           scalar(S_BI_RO);
           hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, r, nr == 0 ? 1 : 0, dst, n);
-          DDA_TRY(matvec(p, av, 0, done));
-          reduce(RED_DOTU, p, av, done);
+          DDA_TRY(matvec(p, av, 0, DONE));
+          reduce(RED_DOTU, p, av, DONE);
           scalar(S_BI_MU);
           hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
           scalar(S_END_ITER);
@@ -872,50 +996,54 @@ struct Core {
       return DDA_OK;
     };
 
-    // Chunks of iterations are enqueued ahead of the flag reads: chunk c+1 is
-    // queued before the state copied after chunk c is examined, so the GPU
-    // never waits on the host, and the chunk length follows the observed
+    // Chunks of iterations are enqueued ahead of the state reads: chunk c+1 is
+    // queued before the states copied after chunk c are examined, so the GPU
+    // never waits on the host; the chunk length follows the slowest observed
     // convergence rate so that few iterations are queued past the one that
     // meets the tolerance (those run as no-ops).
     hipEvent_t ev[2];
     DDA_TRY_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     DDA_TRY_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    DDA_TRY_HIP(hipMemcpyAsync(&host2[0], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipMemcpyAsync(host2, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));
     DDA_TRY_HIP(hipEventRecord(ev[0], st));
     int next_len = 2, chunk = 0;
-    double prev_r = -1;
-    uint64_t prev_count = 0;
+    std::vector<double> prev_r(k, -1.0);
+    std::vector<uint64_t> prev_count(k, 0);
     bool finished = false;
     dda_status rc = DDA_OK;
     while (!finished && rc == DDA_OK) {
-      for (int k = 0; k < next_len && rc == DDA_OK; k++) rc = enqueue_iteration();
+      for (int it = 0; it < next_len && rc == DDA_OK; it++) rc = enqueue_iteration();
       if (rc != DDA_OK) break;
       const int slot = (chunk + 1) & 1;
-      DDA_TRY_HIP(hipMemcpyAsync(&host2[slot], dst, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+      DDA_TRY_HIP(hipMemcpyAsync(host2 + slot * cap, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));
       DDA_TRY_HIP(hipEventRecord(ev[slot], st));
       const int prev = chunk & 1;
       DDA_TRY_HIP(hipEventSynchronize(ev[prev]));
-      const SolverState& hs = host2[prev];
-      if (hs.done) {
-        finished = true;
-      } else {
-        int len = kChunk;
-        if (prev_r > 0 && hs.inprodR > 0 && hs.count > prev_count && hs.inprodR < prev_r && hs.epsB > 0) {
-          const double rate = std::log(prev_r / hs.inprodR) / (double)(hs.count - prev_count);
-          const double need = std::log(hs.inprodR / hs.epsB) / rate - (double)next_len;
-          len = need < 1 ? 1 : (need > kChunk ? kChunk : (int)std::ceil(need));
+      const SolverState* hs = host2 + prev * cap;
+      finished = true;
+      int len = 1;
+      for (int i = 0; i < k; i++) {
+        if (hs[i].done) continue;
+        finished = false;
+        int li = kChunk;
+        if (prev_r[i] > 0 && hs[i].inprodR > 0 && hs[i].count > prev_count[i] && hs[i].inprodR < prev_r[i] &&
+            hs[i].epsB > 0) {
+          const double rate = std::log(prev_r[i] / hs[i].inprodR) / (double)(hs[i].count - prev_count[i]);
+          const double need = std::log(hs[i].inprodR / hs[i].epsB) / rate - (double)next_len;
+          li = need < 1 ? 1 : (need > kChunk ? kChunk : (int)std::ceil(need));
         }
-        prev_r = hs.inprodR;
-        prev_count = hs.count;
-        next_len = len;
+        prev_r[i] = hs[i].inprodR;
+        prev_count[i] = hs[i].count;
+        len = std::max(len, li);
       }
+      next_len = len;
       chunk++;
     }
     (void)hipStreamSynchronize(st);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     if (rc != DDA_OK) return rc;
-    DDA_TRY_HIP(hipMemcpy(fin, dst, sizeof(SolverState), hipMemcpyDeviceToHost));
+    DDA_TRY_HIP(hipMemcpy(fin, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost));
     return DDA_OK;
   }
 };
@@ -934,14 +1062,17 @@ struct dda_solver_s {
   int mode = DDA_MODE_FP64;
   uint64_t max_iter = 0, max_res_increase = 0;
   long long n = 0;
+  int cap = 0;                   // right-hand sides allocated
   Core<double> c64;              // FP64 mode
   Core<float> c32;               // mixed mode inner solver
-  double2* x64 = nullptr;        // mixed mode: FP64 b, x, r, t
+  double2* x64 = nullptr;        // [cap][n] FP64 vectors
   double2* b64 = nullptr;
   double2* r64 = nullptr;
   double2* t64 = nullptr;
-  double* part64 = nullptr;
-  double2* io = nullptr;         // FP64 staging for einc / start / result
+  double* part64 = nullptr;      // [cap][4][nb]
+  double* scale = nullptr;       // [cap] (mixed)
+  int* mask = nullptr;           // [cap] (mixed)
+  double2* io = nullptr;         // [2][cap][n] FP64 staging for einc / start / result
   uint64_t last_iters = 0;
   std::vector<double> outer_hist;
   double inner_floor = 1e-4;
@@ -949,129 +1080,251 @@ struct dda_solver_s {
 
 namespace {
 
-dda_status true_residual64(dda_solver_s* s, double2* x, double2* b, double* rel2, double temp) {
-  Core<double>& K = s->c64;
-  (void)K;
-  hipStream_t st = s->op->ctx->stream;
-  DDA_TRY(launch_matvec<double>(s->op, x, s->t64, 0, st, nullptr));
-  hipLaunchKernelGGL(k_resid64, dim3(kRedBlocks), dim3(kRedThreads), 0, st, s->r64, b, s->t64, s->n, s->part64);
-  DDA_TRY_HIP(hipGetLastError());
-  std::vector<double> h(kRedBlocks);
-  DDA_TRY_HIP(hipMemcpyAsync(h.data(), s->part64, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+// Host sums of the per-RHS partials: quantity 0 into nrm; with rho, also
+// |sum q1 + i q2| / q0 (|r^T r| / |r|^2 after k_resid64 / k_norm64).
+dda_status read_sums(hipStream_t st, const double* part, int nb, int k, std::vector<double>* nrm,
+                     std::vector<double>* rho = nullptr) {
+  std::vector<double> h((size_t)k * 4 * nb);
+  DDA_TRY_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   DDA_TRY_HIP(hipStreamSynchronize(st));
-  double t = 0;
-  for (double v : h) t += v;
-  *rel2 = t / temp;
+  nrm->assign(k, 0.0);
+  if (rho) rho->assign(k, 0.0);
+  for (int r = 0; r < k; r++) {
+    double q[3] = {0, 0, 0};
+    for (int j = 0; j < (rho ? 3 : 1); j++)
+      for (int i = 0; i < nb; i++) q[j] += h[((size_t)r * 4 + j) * nb + i];
+    (*nrm)[r] = q[0];
+    if (rho) (*rho)[r] = q[0] > 0 ? std::hypot(q[1], q[2]) / q[0] : 0.0;
+  }
   return DDA_OK;
 }
 
-dda_status run_fp64(dda_solver_s* s, double eps, bool have_start, dda_solve_info* info) {
+dda_status alloc_buffers(dda_solver_s* s, int k) {
+  if (k <= s->cap) return DDA_OK;
+  dda_op_s* op = s->op;
+  DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  void* bufs[] = {s->x64, s->b64, s->r64, s->t64, s->part64, s->scale, s->mask, s->io};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  s->x64 = s->b64 = s->r64 = s->t64 = s->io = nullptr;
+  s->part64 = s->scale = nullptr;
+  s->mask = nullptr;
+  const int nb = red_blocks(s->n);
+  const size_t vb = (size_t)k * s->n * sizeof(double2);
+  DDA_TRY_HIP(hipMalloc(&s->io, 2 * vb));
+  DDA_TRY_HIP(hipMalloc(&s->t64, vb));
+  DDA_TRY_HIP(hipMalloc(&s->r64, vb));
+  DDA_TRY_HIP(hipMalloc(&s->part64, (size_t)k * 4 * nb * sizeof(double)));
+  DDA_TRY_HIP(hipMalloc(&s->scale, k * sizeof(double)));
+  DDA_TRY_HIP(hipMalloc(&s->mask, k * sizeof(int)));
+  if (s->mode == DDA_MODE_MIXED) {
+    DDA_TRY_HIP(hipMalloc(&s->x64, vb));
+    DDA_TRY_HIP(hipMalloc(&s->b64, vb));
+    DDA_TRY(s->c32.alloc(s->op32, s->kind, s->max_iter, s->max_res_increase, k));
+  } else {
+    DDA_TRY(s->c64.alloc(op, s->kind, s->max_iter, s->max_res_increase, k));
+  }
+  s->cap = k;
+  return DDA_OK;
+}
+
+// |b - A x|^2 per right-hand side (FP64 block matvec into t64, r64 = b - t64);
+// right-hand sides with skip[r] != 0 are left out (their sums are undefined).
+dda_status residual64(dda_solver_s* s, int k, double2* x, double2* b, const std::vector<int>& skip,
+                      std::vector<double>* rr, std::vector<double>* rho = nullptr) {
+  hipStream_t st = s->op->ctx->stream;
+  const int nb = red_blocks(s->n);
+  DDA_TRY_HIP(hipMemcpyAsync(s->mask, skip.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+  DDA_TRY(launch_matvec<double>(s->op, x, s->t64, 0, st, s->mask, k, 1));
+  hipLaunchKernelGGL(k_resid64, dim3(nb, k), dim3(kRedThreads), 0, st, s->r64, b, s->t64, s->n, s->part64, s->mask);
+  DDA_TRY_HIP(hipGetLastError());
+  return read_sums(st, s->part64, nb, k, rr, rho);
+}
+
+void fill_info(dda_solve_info* info, int r, uint64_t iters, uint64_t mvs, uint64_t refine, double rel, double trel,
+               int status) {
+  if (!info) return;
+  info[r].iterations = iters;
+  info[r].matvecs = mvs;
+  info[r].refinements = refine;
+  info[r].rel_residual = rel;
+  info[r].true_rel_residual = trel;
+  info[r].status = status;
+}
+
+dda_status run_fp64(dda_solver_s* s, int k, double eps, bool have_start, dda_solve_info* info) {
   dda_op_s* op = s->op;
   hipStream_t st = op->ctx->stream;
   const MvDims& d = op->dims;
   Core<double>& K = s->c64;
   const double2* ccs = (const double2*)op->ccs;
   const double gu = op->grid_unit;
-  hipLaunchKernelGGL(k_setup_b<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, K.v[0], s->io, ccs, op->mat_dev,
-                     d.nv, d.S, K.part);
+  const dim3 G(red_blocks(s->n), k), B(kRedThreads);
+  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, K.v[0], s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat);
   if (have_start)
-    hipLaunchKernelGGL(k_setup_start<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, K.v[2], s->io + s->n, ccs,
-                       op->mat_dev, d.nv, d.S, 1.0 / (gu * gu * gu));
+    hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, K.v[2], s->io + (size_t)s->cap * s->n, ccs, op->mat_dev,
+                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu));
   DDA_TRY_HIP(hipGetLastError());
-  SolverState fin;
-  DDA_TRY(K.solve(eps, have_start, &fin));
-  s->last_iters = fin.count;
-  hipLaunchKernelGGL(k_get_result<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, s->io, K.v[2], ccs,
-                     op->mat_dev, d.nv, d.S, gu * gu * gu);
+  std::vector<SolverState> fin(k);
+  std::vector<double> epsv(k, eps);
+  DDA_TRY(K.solve(k, epsv.data(), nullptr, have_start, fin.data()));
+  hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, K.v[2], ccs, op->mat_dev, d.nv, d.S, d.nmat,
+                     gu * gu * gu);
   DDA_TRY_HIP(hipGetLastError());
-  double rel2 = 0;
-  DDA_TRY(true_residual64(s, K.v[2], K.v[0], &rel2, fin.temp));
-  if (info) {
-    info->iterations = fin.count;
-    info->rel_residual = std::sqrt(fin.residScale * fin.inprodR);
-    info->true_rel_residual = std::sqrt(rel2);
-    info->refinements = 0;
-    const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
-    info->matvecs = fin.count * per + 2;
+  std::vector<double> rr;
+  std::vector<int> none(k, 0);
+  for (int r = 0; r < k; r++) none[r] = fin[r].temp > 0 ? 0 : 1;
+  DDA_TRY(residual64(s, k, K.v[2], K.v[0], none, &rr));
+  dda_status worst = DDA_OK;
+  s->last_iters = fin[0].count;
+  const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
+  for (int r = 0; r < k; r++) {
+    fill_info(info, r, fin[r].count, fin[r].count * per + 2, 0, std::sqrt(fin[r].residScale * fin[r].inprodR),
+              none[r] ? 0.0 : std::sqrt(rr[r] / fin[r].temp), fin[r].status);
+    if (fin[r].status != DDA_OK && worst == DDA_OK) worst = (dda_status)fin[r].status;
   }
-  return (dda_status)fin.status;
+  return worst;
 }
 
-// FP64 iterative refinement around an FP32 Krylov solve:
-//   r = b - A64 x; stop when |r|^2 < eps^2 |b|^2; solve A32 d = r / |r| to
+// FP64 iterative refinement around FP32 Krylov solves, per right-hand side:
+//   r = b - A64 x; stop when |r|^2 < eps^2 |b|^2; solve A d = r / |r| to
 //   max(floor, eps |b| / (2 |r|)); x += |r| d.
-dda_status run_mixed(dda_solver_s* s, double eps, bool have_start, dda_solve_info* info) {
+// The inner solve runs in FP32 unless it cannot make progress there, when it
+// runs on the FP64 operator for that right-hand side and step:
+//   - QMR-CS and BiCG-CS build their Lanczos basis with the unconjugated
+//     product; when |r^T r| / |r|^2 < kRhoMin the first basis vector is
+//     r / sqrt(r^T r) and FP32 rounding, amplified by |r|^2 / |r^T r|, stalls
+//     the iteration (an incident wave whose phase pattern nearly cancels in
+//     sum_i E_i^2 over the particle does this);
+//   - after an FP32 step that did not halve the FP64 residual, that
+//     right-hand side stays on FP64 for the rest of the solve.
+// Right-hand sides that have converged are masked out of the later steps.
+constexpr double kRhoMin = 1e-4;
+
+dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_solve_info* info) {
   dda_op_s* op = s->op;
   hipStream_t st = op->ctx->stream;
   const MvDims& d = op->dims;
-  Core<float>& K = s->c32;
   const double2* ccs = (const double2*)op->ccs;
   const double gu = op->grid_unit;
-  const dim3 G(kRedBlocks), B(kRedThreads);
-  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->b64, s->io, ccs, op->mat_dev, d.nv, d.S, s->part64);
+  const int nb = red_blocks(s->n);
+  const dim3 G(nb, k), B(kRedThreads);
+  const bool lanczos = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS;
+  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->b64, s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat);
+  hipLaunchKernelGGL(k_norm64, G, B, 0, st, s->b64, s->n, s->part64);
   DDA_TRY_HIP(hipGetLastError());
-  std::vector<double> h(kRedBlocks);
-  DDA_TRY_HIP(hipMemcpyAsync(h.data(), s->part64, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
-  DDA_TRY_HIP(hipStreamSynchronize(st));
-  double temp = 0;
-  for (double v : h) temp += v;
-  if (!(temp > 0)) {
-    set_error("zero right-hand side");
-    return DDA_INVALID_ARG;
-  }
+  std::vector<double> temp, rho;
+  DDA_TRY(read_sums(st, s->part64, nb, k, &temp, &rho));
+  std::vector<int> status(k, DDA_OK), conv(k, 0), sticky64(k, 0);
+  for (int r = 0; r < k; r++)
+    if (!(temp[r] > 0)) {
+      status[r] = DDA_INVALID_ARG;  // a zero right-hand side
+      conv[r] = 1;
+    }
   if (have_start)
-    hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, s->x64, s->io + s->n, ccs, op->mat_dev, d.nv, d.S,
-                       1.0 / (gu * gu * gu));
+    hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, s->x64, s->io + (size_t)s->cap * s->n, ccs, op->mat_dev,
+                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu));
   else
-    DDA_TRY_HIP(hipMemsetAsync(s->x64, 0, s->n * sizeof(double2), st));
+    DDA_TRY_HIP(hipMemsetAsync(s->x64, 0, (size_t)k * s->n * sizeof(double2), st));
   s->outer_hist.clear();
-  uint64_t iters = 0, outer = 0, matvecs = 0;
-  dda_status status = DDA_OK;
-  double rel2 = 1.0;
+  std::vector<uint64_t> iters(k, 0), mvs(k, 0), outer(k, 0);
+  std::vector<double> rel2(k, 1.0), rr;
+  const uint64_t per = lanczos ? 1 : 2;
   const uint64_t max_outer = 40;
-  for (;;) {
-    if (outer == 0 && !have_start) {
-      DDA_TRY_HIP(hipMemcpyAsync(s->r64, s->b64, s->n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-      rel2 = 1.0;
+  for (uint64_t step = 0;; step++) {
+    std::vector<double> prev = rel2;
+    if (step == 0 && !have_start) {
+      DDA_TRY_HIP(hipMemcpyAsync(s->r64, s->b64, (size_t)k * s->n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      for (int r = 0; r < k; r++) rel2[r] = 1.0;
     } else {
-      DDA_TRY(true_residual64(s, s->x64, s->b64, &rel2, temp));
-      matvecs++;
+      DDA_TRY(residual64(s, k, s->x64, s->b64, conv, &rr, &rho));
+      for (int r = 0; r < k; r++) {
+        if (conv[r]) continue;
+        rel2[r] = rr[r] / temp[r];
+        mvs[r]++;
+      }
     }
-    s->outer_hist.push_back(rel2);
-    if (rel2 < eps * eps) break;
-    if (outer >= max_outer || iters >= s->max_iter) {
-      status = DDA_MAX_ITER;
-      break;
+    s->outer_hist.push_back(rel2[0]);
+    bool any = false;
+    std::vector<double> eps_in(k, 1.0), sc(k, 0.0);
+    std::vector<int> act32(k, 0), act64(k, 0);
+    for (int r = 0; r < k; r++) {
+      if (conv[r]) continue;
+      if (rel2[r] < eps * eps) {
+        conv[r] = 1;
+        continue;
+      }
+      if (outer[r] >= max_outer || iters[r] >= s->max_iter) {
+        status[r] = DDA_MAX_ITER;
+        conv[r] = 1;
+        continue;
+      }
+      if (outer[r] > 0 && !sticky64[r] && rel2[r] > 0.25 * prev[r]) sticky64[r] = 1;
+      any = true;
+      (sticky64[r] || (lanczos && rho[r] < kRhoMin) ? act64 : act32)[r] = 1;
+      sc[r] = 1.0 / std::sqrt(rel2[r] * temp[r]);
+      eps_in[r] = std::max(s->inner_floor, 0.5 * eps / std::sqrt(rel2[r]));
     }
-    const double rn = std::sqrt(rel2 * temp);
-    hipLaunchKernelGGL(k_to_f32, G, B, 0, st, (float2*)K.v[0], s->r64, s->n, 1.0 / rn);
-    DDA_TRY_HIP(hipGetLastError());
-    const double eps_in = std::max(s->inner_floor, 0.5 * eps / std::sqrt(rel2));
-    K.max_iter = s->max_iter - iters;
-    SolverState fin;
-    DDA_TRY(K.solve(eps_in, false, &fin));
-    iters += fin.count;
-    const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
-    matvecs += fin.count * per + 1;
-    outer++;
-    if (fin.status == DDA_BREAKDOWN && fin.count == 0) {
-      status = DDA_BREAKDOWN;
-      break;
+    if (!any) break;
+    std::vector<double> inv(k, 0.0);
+    for (int r = 0; r < k; r++) inv[r] = sc[r] > 0 ? 1.0 / sc[r] : 0.0;
+    for (int pass = 0; pass < 2; pass++) {
+      const std::vector<int>& act = pass == 0 ? act32 : act64;
+      bool some = false;
+      for (int r = 0; r < k; r++) some = some || act[r];
+      if (!some) continue;
+      std::vector<int> msk(k);
+      std::vector<uint64_t> budget(k, 0);
+      for (int r = 0; r < k; r++) {
+        msk[r] = act[r] ? 0 : 1;
+        if (act[r]) budget[r] = s->max_iter - iters[r];
+      }
+      DDA_TRY_HIP(hipMemcpyAsync(s->scale, sc.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
+      DDA_TRY_HIP(hipMemcpyAsync(s->mask, msk.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+      std::vector<SolverState> fin(k);
+      if (pass == 0) {
+        hipLaunchKernelGGL(k_load_rhs<float2>, G, B, 0, st, (float2*)s->c32.v[0], s->r64, s->n, s->scale, s->mask);
+        DDA_TRY_HIP(hipGetLastError());
+        DDA_TRY(s->c32.solve(k, eps_in.data(), act.data(), false, fin.data(), budget.data()));
+      } else {
+        if (s->c64.cap < s->cap) DDA_TRY(s->c64.alloc(op, s->kind, s->max_iter, s->max_res_increase, s->cap));
+        hipLaunchKernelGGL(k_load_rhs<double2>, G, B, 0, st, s->c64.v[0], s->r64, s->n, s->scale, s->mask);
+        DDA_TRY_HIP(hipGetLastError());
+        DDA_TRY(s->c64.solve(k, eps_in.data(), act.data(), false, fin.data(), budget.data()));
+      }
+      for (int r = 0; r < k; r++) {
+        if (!act[r]) continue;
+        iters[r] += fin[r].count;
+        mvs[r] += fin[r].count * per + 1;
+        if (fin[r].status == DDA_BREAKDOWN && fin[r].count == 0) {
+          status[r] = DDA_BREAKDOWN;
+          conv[r] = 1;
+          msk[r] = 1;
+        }
+      }
+      DDA_TRY_HIP(hipMemcpyAsync(s->scale, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
+      DDA_TRY_HIP(hipMemcpyAsync(s->mask, msk.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+      if (pass == 0)
+        hipLaunchKernelGGL(k_add_corr<float2>, G, B, 0, st, s->x64, (const float2*)s->c32.v[2], s->n, s->scale,
+                           s->mask);
+      else
+        hipLaunchKernelGGL(k_add_corr<double2>, G, B, 0, st, s->x64, (const double2*)s->c64.v[2], s->n, s->scale,
+                           s->mask);
+      DDA_TRY_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_add_f32, G, B, 0, st, s->x64, (const float2*)K.v[2], s->n, rn);
-    DDA_TRY_HIP(hipGetLastError());
+    for (int r = 0; r < k; r++)
+      if (act32[r] || act64[r]) outer[r]++;
   }
-  s->last_iters = iters;
-  hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, s->x64, ccs, op->mat_dev, d.nv, d.S, gu * gu * gu);
+  s->last_iters = iters[0];
+  hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, s->x64, ccs, op->mat_dev, d.nv, d.S, d.nmat,
+                     gu * gu * gu);
   DDA_TRY_HIP(hipGetLastError());
-  if (info) {
-    info->iterations = iters;
-    info->rel_residual = std::sqrt(rel2);
-    info->true_rel_residual = std::sqrt(rel2);
-    info->refinements = outer;
-    info->matvecs = matvecs;
+  dda_status worst = DDA_OK;
+  for (int r = 0; r < k; r++) {
+    fill_info(info, r, iters[r], mvs[r], outer[r], std::sqrt(rel2[r]), std::sqrt(rel2[r]), status[r]);
+    if (status[r] != DDA_OK && worst == DDA_OK) worst = (dda_status)status[r];
   }
-  return status;
+  return worst;
 }
 
 }  // namespace
@@ -1079,7 +1332,7 @@ dda_status run_mixed(dda_solver_s* s, double eps, bool have_start, dda_solve_inf
 static void free_solver(dda_solver_s* s) {
   s->c64.release();
   s->c32.release();
-  void* bufs[] = {s->x64, s->b64, s->r64, s->t64, s->part64, s->io};
+  void* bufs[] = {s->x64, s->b64, s->r64, s->t64, s->part64, s->scale, s->mask, s->io};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (s->op32) dda_op_destroy(s->op32);
@@ -1107,42 +1360,25 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
   s->max_res_increase = max_res_increase ? max_res_increase : defaults[kind];
   s->n = 3ll * d.S;
   dda_status rc = DDA_OK;
-  auto dm = [&](void** p, size_t bytes) {
-    if (rc != DDA_OK) return;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-      set_error(std::string("dda_solver_create: ") + hipGetErrorString(e));
-      rc = e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
-    }
-  };
-  dm((void**)&s->io, 2 * s->n * sizeof(double2));
-  dm((void**)&s->t64, s->n * sizeof(double2));
-  dm((void**)&s->r64, s->n * sizeof(double2));
-  dm((void**)&s->part64, 4 * kRedBlocks * sizeof(double));
-  if (rc == DDA_OK && mode == DDA_MODE_FP64) rc = s->c64.alloc(op, kind, s->max_iter, s->max_res_increase);
-  if (rc == DDA_OK && mode == DDA_MODE_MIXED) {
-    dm((void**)&s->x64, s->n * sizeof(double2));
-    dm((void**)&s->b64, s->n * sizeof(double2));
-    if (rc == DDA_OK) {
-      dda_op_desc desc;
-      std::memset(&desc, 0, sizeof desc);
-      desc.box[0] = d.Bx;
-      desc.box[1] = d.By;
-      desc.box[2] = d.Bz;
-      desc.grid[0] = d.Gx;
-      desc.grid[1] = d.Gy;
-      desc.grid[2] = d.Gz;
-      desc.nv = d.nv;
-      desc.vec_stride = d.S;
-      desc.pos = op->pos.data();
-      desc.mat = op->mat.data();
-      desc.nmat = op->nmat;
-      desc.grid_unit = op->grid_unit;
-      desc.wave_num = op->wave_num;
-      rc = dda_op_create(op->ctx, &desc, 32, &s->op32);
-    }
-    if (rc == DDA_OK) rc = s->c32.alloc(s->op32, kind, s->max_iter, s->max_res_increase);
+  if (mode == DDA_MODE_MIXED) {
+    dda_op_desc desc;
+    std::memset(&desc, 0, sizeof desc);
+    desc.box[0] = d.Bx;
+    desc.box[1] = d.By;
+    desc.box[2] = d.Bz;
+    desc.grid[0] = d.Gx;
+    desc.grid[1] = d.Gy;
+    desc.grid[2] = d.Gz;
+    desc.nv = d.nv;
+    desc.vec_stride = d.S;
+    desc.pos = op->pos.data();
+    desc.mat = op->mat.data();
+    desc.nmat = op->nmat;
+    desc.grid_unit = op->grid_unit;
+    desc.wave_num = op->wave_num;
+    rc = dda_op_create(op->ctx, &desc, 32, &s->op32);
   }
+  if (rc == DDA_OK) rc = alloc_buffers(s, 1);
   if (rc != DDA_OK) {
     free_solver(s);
     return rc;
@@ -1163,35 +1399,67 @@ extern "C" dda_status dda_solver_destroy(dda_solver_t s) {
   return DDA_OK;
 }
 
-extern "C" dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, const double* start,
-                                       double* pol_out, dda_solve_info* info) {
-  if (!s || !einc || !pol_out || !(eps > 0)) {
-    set_error("dda_solver_solve: null argument or eps <= 0");
+// A zero right-hand side has no solution to report: its output is zero.
+static dda_status zero_invalid(dda_solver_s* s, int k, const dda_solve_info* info) {
+  for (int r = 0; r < k; r++)
+    if (info[r].status == DDA_INVALID_ARG)
+      DDA_TRY_HIP(hipMemsetAsync(s->io + (size_t)r * s->n, 0, s->n * sizeof(double2), s->op->ctx->stream));
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, double eps,
+                                             const double* start, double* pol_out, dda_solve_info* info) {
+  if (!s || !einc || !pol_out || !(eps > 0) || nrhs < 1) {
+    set_error("dda_solver_solve: null argument, eps <= 0 or nrhs < 1");
     return DDA_INVALID_ARG;
   }
   dda_op_s* op = s->op;
   if (!op->cc_set) {
-    set_error("dda_solver_solve: dda_op_set_cc_sqrt has not been called");
+    set_error("dda_solver_solve: couple constants have not been set");
     return DDA_INVALID_ARG;
+  }
+  if (op->cc_rhs < nrhs) {
+    if (op->cc_rhs != 1) {
+      set_error("dda_solver_solve: couple constants set for fewer right-hand sides than solved");
+      return DDA_INVALID_ARG;
+    }
+    // one set of couple constants for every right-hand side
+    std::vector<double> rep;
+    for (int r = 0; r < nrhs; r++) rep.insert(rep.end(), op->cc_sqrt_host.begin(), op->cc_sqrt_host.end());
+    DDA_TRY(dda_op_set_cc_sqrt_batch(op, rep.data(), nrhs));
   }
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = op->ctx->stream;
+  DDA_TRY(alloc_buffers(s, nrhs));
   if (s->mode == DDA_MODE_MIXED) {
-    DDA_TRY(dda_op_set_cc_sqrt(s->op32, op->cc_sqrt_host.data()));
+    DDA_TRY(dda_op_set_cc_sqrt_batch(s->op32, op->cc_sqrt_host.data(), nrhs));
     s->op32->prof = op->prof;
   }
-  DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, s->n * sizeof(double2), hipMemcpyHostToDevice, st));
-  if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + s->n, start, s->n * sizeof(double2), hipMemcpyHostToDevice, st));
-  dda_status rc = s->mode == DDA_MODE_MIXED ? run_mixed(s, eps, start != nullptr, info)
-                                            : run_fp64(s, eps, start != nullptr, info);
-  if (rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_INVALID_ARG || rc == DDA_NOT_SUPPORTED) return rc;
-  DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, s->n * sizeof(double2), hipMemcpyDeviceToHost, st));
+  const size_t bytes = (size_t)nrhs * s->n * sizeof(double2);
+  DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, bytes, hipMemcpyHostToDevice, st));
+  if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + (size_t)s->cap * s->n, start, bytes, hipMemcpyHostToDevice, st));
+  std::vector<dda_solve_info> inf(nrhs);
+  dda_status rc = s->mode == DDA_MODE_MIXED ? run_mixed(s, nrhs, eps, start != nullptr, inf.data())
+                                            : run_fp64(s, nrhs, eps, start != nullptr, inf.data());
+  if (rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED) return rc;
+  DDA_TRY(zero_invalid(s, nrhs, inf.data()));
+  DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, bytes, hipMemcpyDeviceToHost, st));
   DDA_TRY_HIP(hipStreamSynchronize(st));
-  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int r = 0; r < nrhs; r++) {
+    inf[r].seconds = secs;
+    if (info) info[r] = inf[r];
+  }
   if (rc == DDA_BREAKDOWN) set_error("solver breakdown");
   if (rc == DDA_MAX_ITER) set_error("Too many iterations");
   if (rc == DDA_STAGNATION) set_error("Too many iterations w/o increase");
+  if (rc == DDA_INVALID_ARG) set_error("a right-hand side is zero");
   return rc;
+}
+
+extern "C" dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, const double* start,
+                                       double* pol_out, dda_solve_info* info) {
+  return dda_solver_solve_batch(s, einc, 1, eps, start, pol_out, info);
 }
 
 extern "C" dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n, uint64_t* written) {

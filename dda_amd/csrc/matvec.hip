@@ -33,7 +33,7 @@ namespace dda {
 // Kernels
 // ------------------------------------------------------------------------------------
 
-__device__ inline bool stop_requested(const int* done) { return done != nullptr && *done != 0; }
+__device__ inline bool stopped(const RhsFlags& f, int rhs) { return f.done != nullptr && f.done[rhs * f.stride] != 0; }
 
 // Pass 1: rows r0 .. r0+Lr-1 of the (z, y) plane, all three components. Tile
 // line l = c * Lr + lr.
@@ -45,10 +45,13 @@ __global__ __launch_bounds__(256) void k_pass1(const typename CxT<R>::T* __restr
                                                const uint32_t* __restrict__ row_meta,
                                                const typename CxT<R>::T* __restrict__ ccs, MvDims d, int lr_n,
                                                FftPlan px, FastDiv ldiv, FastDiv ndiv,
-                                               const typename CxT<R>::T* __restrict__ tw, int conj,
-                                               const int* done) {
+                                               const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  if (stop_requested(done)) return;
+  const int rhs = blockIdx.y;
+  if (stopped(fl, rhs)) return;
+  x += (long long)rhs * 3 * d.S;
+  a1 += (long long)rhs * 3 * d.nrows * d.Gx;
+  ccs += rhs * 3 * d.nmat;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
@@ -99,10 +102,14 @@ __global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restr
                                                const uint32_t* __restrict__ row_meta,
                                                const typename CxT<R>::T* __restrict__ ccs, MvDims d, int lr_n,
                                                FftPlan px, FastDiv ldiv, FastDiv ndiv,
-                                               const typename CxT<R>::T* __restrict__ tw, int conj,
-                                               const int* done) {
+                                               const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  if (stop_requested(done)) return;
+  const int rhs = blockIdx.y;
+  if (stopped(fl, rhs)) return;
+  a1 += (long long)rhs * 3 * d.nrows * d.Gx;
+  x += (long long)rhs * 3 * d.S;
+  y += (long long)rhs * 3 * d.S;
+  ccs += rhs * 3 * d.nmat;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
@@ -165,9 +172,9 @@ struct ColParams {
 template <typename R, int PPT, bool INV>
 __global__ __launch_bounds__(256) void k_colfft(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
                                                 FftPlan p, FastDiv tdiv, const typename CxT<R>::T* __restrict__ tw,
-                                                const int* done) {
+                                                RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  if (stop_requested(done)) return;
+  if (stopped(fl, blockIdx.y / fl.per)) return;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int T = (int)tdiv.d;
@@ -209,16 +216,18 @@ template <typename R, int PPT>
 __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
                                                MvDims d, FftPlan pz, FastDiv tdiv,
                                                const typename CxT<R>::T* __restrict__ tw, int nwork, int nper,
-                                               const int* done) {
+                                               int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  if (stop_requested(done)) return;
   const int T = (int)tdiv.d;
   const int ntx = (d.Gx + T - 1) / T;
   const int ftn = (ntx + 1) / 2;
   const int hx = d.Gx / 2, hy = d.Gy / 2, hz = d.Gz / 2;
   const int w = (blockIdx.x & 7) * nper + (blockIdx.x >> 3);
   if (w >= nwork) return;
-  const int sub = w & 3, grp = w >> 2;
+  const int sub = w & 3, q = w >> 2;
+  const int rhs = q % nrhs, grp = q / nrhs;
+  if (stopped(fl, rhs)) return;
+  a2 += (long long)rhs * 3 * d.Bz * d.Gy * d.Gx;
   const int ft = grp % ftn, fyp = grp / ftn;
   int ky = fyp, tile_x = ft;
   if (sub & 2) {
@@ -318,19 +327,21 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
 template <typename R, int N, int T>
 __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
-                                                int nper, const int* done) {
+                                                int nper, int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
   typedef StaticPlan<N, sizeof(R) == 4> P;
   constexpr int PPT = P::PPT;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Inv;
-  if (stop_requested(done)) return;
   const int ntx = (d.Gx + T - 1) / T;
   const int ftn = (ntx + 1) / 2;
   const int hx = d.Gx / 2, hy = d.Gy / 2, hz = N / 2;
   const int w = (blockIdx.x & 7) * nper + (blockIdx.x >> 3);
   if (w >= nwork) return;
-  const int sub = w & 3, grp = w >> 2;
+  const int sub = w & 3, q = w >> 2;
+  const int rhs = q % nrhs, grp = q / nrhs;
+  if (stopped(fl, rhs)) return;
+  a2 += (long long)rhs * 3 * d.Bz * d.Gy * d.Gx;
   const int ft = grp % ftn, fyp = grp / ftn;
   int ky = fyp, tile_x = ft;
   if (sub & 2) {
@@ -395,13 +406,13 @@ __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const ty
 // Column FFT with a static plan (pass 2 / pass 4 and the tensor generation).
 template <typename R, int N, int T, bool INV>
 __global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
-                                                  const typename CxT<R>::T* __restrict__ tw, const int* done) {
+                                                  const typename CxT<R>::T* __restrict__ tw, RhsFlags fl) {
   typedef typename CxT<R>::T C;
   typedef StaticPlan<N, sizeof(R) == 4> P;
   constexpr int PPT = P::PPT;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Out;
-  if (stop_requested(done)) return;
+  if (stopped(fl, blockIdx.y / fl.per)) return;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   constexpr int lp = (T & 1) ? T : T + 1;
@@ -577,7 +588,7 @@ dda_status set_lds(K kernel, size_t bytes) {
 
 template <typename R, bool INV>
 dda_status launch_col(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
-                      int nbatch, hipStream_t st, const int* done) {
+                      int nbatch, hipStream_t st, RhsFlags done) {
   typedef typename CxT<R>::T C;
   const int pts = p.n * T;
   const size_t lds = (size_t)p.n * lds_stride(T) * sizeof(C);
@@ -641,7 +652,7 @@ constexpr int static_cols() {
 
 template <typename R, int N, bool INV>
 dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
-                             const int* done) {
+                             RhsFlags done) {
   typedef typename CxT<R>::T C;
   constexpr int T = static_cols<R, N>();
   constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
@@ -656,7 +667,7 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
 }
 
 template <typename R, int N>
-dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, const int* done) {
+dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
   constexpr int T = static_cols<R, N>();
   constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
@@ -664,12 +675,12 @@ dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, const int* done) {
   const MvDims& d = op->dims;
   const size_t lds = (size_t)N * lp * sizeof(C);
   const int ntx = (d.Gx + T - 1) / T;
-  const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4;
+  const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
   const int nper = (nwork + 7) / 8;
   auto k = k_pass3s<R, N, T>;
   DDA_TRY(set_lds(k, lds));
   hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)op->a2, (const C*)op->dhat, d,
-                     (const C*)op->tw[2], nwork, nper, done);
+                     (const C*)op->tw[2], nwork, nper, nrhs, done);
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
 }
@@ -690,7 +701,7 @@ inline bool has_static_plan(int n) {
 
 template <typename R, bool INV>
 dda_status launch_col_any(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
-                          int nbatch, hipStream_t st, const int* done) {
+                          int nbatch, hipStream_t st, RhsFlags done) {
   switch (p.n) {
 #define X(N) \
   case N:    \
@@ -739,8 +750,23 @@ struct PassTimer {
 };
 
 template <typename R>
-dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done) {
+dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done_ptr,
+                         int nrhs, int done_stride) {
   typedef typename CxT<R>::T C;
+  if (nrhs < 1 || nrhs > op->cap_rhs) {
+    set_error("matvec: number of right-hand sides exceeds the operator's capacity");
+    return DDA_INVALID_ARG;
+  }
+  if (3ll * op->dims.Bz * nrhs > 65535) {
+    set_error("matvec: 3 * box z * right-hand sides exceeds the 65535 column batches of one launch");
+    return DDA_NOT_SUPPORTED;
+  }
+  RhsFlags done;
+  done.done = done_ptr;
+  done.stride = done_stride;
+  done.per = 1;
+  RhsFlags col_done = done;
+  col_done.per = 3 * op->dims.Bz;
   const MvDims& d = op->dims;
   const C* x = (const C*)xv;
   C* y = (C*)yv;
@@ -757,7 +783,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
   {
     PassTimer pt(op, st, 0);
-    dim3 grid((d.nrows + lr - 1) / lr);
+    dim3 grid((d.nrows + lr - 1) / lr, nrhs);
     if (ppt_x == 8) {
       auto k = k_pass1<R, 8>;
       DDA_TRY(set_lds(k, lds_x));
@@ -782,7 +808,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.By * d.Gx;
     cp.bstride_out = (long long)d.Gy * d.Gx;
-    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done)));
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
   if (op->static_z) {
@@ -790,7 +816,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     switch (d.Gz) {
 #define X(N)                                               \
   case N:                                                  \
-    DDA_TRY((launch_pass3_static<R, N>(op, st, done))); \
+    DDA_TRY((launch_pass3_static<R, N>(op, st, done, nrhs))); \
     break;
       DDA_STATIC_SIZES(X)
 #undef X
@@ -803,7 +829,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     const int pts = d.Gz * T;
     const size_t lds = (size_t)d.Gz * lds_stride(T) * sizeof(C);
     const int ntx = (d.Gx + T - 1) / T;
-    const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4;
+    const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
     const int nper = (nwork + 7) / 8;
     dim3 grid(8 * nper);
     FastDiv tdiv = make_fastdiv(T);
@@ -811,12 +837,12 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
       auto k = k_pass3<R, 8>;
       DDA_TRY(set_lds(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], nwork, nper, done);
+                         (const C*)op->tw[2], nwork, nper, nrhs, done);
     } else if (pts <= 16 * 256) {
       auto k = k_pass3<R, 16>;
       DDA_TRY(set_lds(k, lds));
       hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], nwork, nper, done);
+                         (const C*)op->tw[2], nwork, nper, nrhs, done);
     } else {
       set_error("Z pass tile too large");
       return DDA_NOT_SUPPORTED;
@@ -834,12 +860,12 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.Gy * d.Gx;
     cp.bstride_out = (long long)d.By * d.Gx;
-    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz, st, done)));
+    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done)));
   }
   // pass 5
   {
     PassTimer pt(op, st, 4);
-    dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1);
+    dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1, nrhs);
     if (ppt_x == 8) {
       auto k = k_pass5<R, 8>;
       DDA_TRY(set_lds(k, lds_x));
@@ -858,8 +884,8 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   return DDA_OK;
 }
 
-template dda_status launch_matvec<float>(dda_op_s*, const void*, void*, int, hipStream_t, const int*);
-template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hipStream_t, const int*);
+template dda_status launch_matvec<float>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int);
+template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int);
 
 // Generate the octant of the FFT'd, d^3-scaled tensor on the device in FP64:
 // fill one component on the full grid, 3-D FFT (X rows, Y and Z columns), keep
@@ -884,14 +910,14 @@ static dda_status generate_dhat(dda_op_s* op) {
     cy.in_len = cy.out_len = d.Gy;
     cy.pstride_in = cy.pstride_out = d.Gx;
     cy.bstride_in = cy.bstride_out = (long long)d.Gx * d.Gy;
-    rc = launch_col_any<double, false>(w, w, cy, op->plan[1], col_tile(d.Gy, 8), tw64[1], d.Gz, st, nullptr);
+    rc = launch_col_any<double, false>(w, w, cy, op->plan[1], col_tile(d.Gy, 8), tw64[1], d.Gz, st, RhsFlags());
     if (rc != DDA_OK) break;
     ColParams cz;
     cz.ncols = d.Gx * d.Gy;
     cz.in_len = cz.out_len = d.Gz;
     cz.pstride_in = cz.pstride_out = (long long)d.Gx * d.Gy;
     cz.bstride_in = cz.bstride_out = 0;
-    rc = launch_col_any<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, nullptr);
+    rc = launch_col_any<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, RhsFlags());
     if (rc != DDA_OK) break;
     if (op->prec == 64)
       hipLaunchKernelGGL(k_dhat_extract<double>, dim3(1024), dim3(256), 0, st, w, (double2*)op->dhat, comp, d.Gx, d.Gy,
@@ -993,6 +1019,7 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
   d.nv = ds.nv;
   d.S = S;
   d.nrows = d.By * d.Bz;
+  d.nmat = ds.nmat;
   for (int a = 0; a < 3; a++)
     if (!make_plan(grid[a], &op->plan[a])) {
       set_error("dda_op_create: grid sizes must factor into 2, 3, 5, 7");
@@ -1140,43 +1167,100 @@ extern "C" dda_status dda_op_info(dda_op_t op, uint32_t grid[3], uint32_t* vec_s
   return DDA_OK;
 }
 
+// Grid buffers and couple constants for `nrhs` right-hand sides.
+static dda_status ensure_rhs_capacity(dda_op_s* op, int nrhs) {
+  if (nrhs <= op->cap_rhs) return DDA_OK;
+  DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  const MvDims& d = op->dims;
+  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
+  const size_t a1n = 3ull * d.Bz * d.By * d.Gx, a2n = 3ull * d.Bz * d.Gy * d.Gx;
+  void *a1 = nullptr, *a2 = nullptr, *ccs = nullptr;
+  hipError_t e = hipMalloc(&a1, a1n * csz * nrhs);
+  if (e == hipSuccess) e = hipMalloc(&a2, a2n * csz * nrhs);
+  if (e == hipSuccess) e = hipMalloc(&ccs, 3ull * d.nmat * csz * nrhs);
+  if (e == hipSuccess) e = hipMemcpy(ccs, op->ccs, 3ull * d.nmat * csz * op->cap_rhs, hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) {
+    if (a1) (void)hipFree(a1);
+    if (a2) (void)hipFree(a2);
+    if (ccs) (void)hipFree(ccs);
+    set_error(std::string("grid buffers for a batch: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+  }
+  op->device_bytes += (a1n + a2n + 3ull * d.nmat) * csz * (nrhs - op->cap_rhs);
+  (void)hipFree(op->a1);
+  (void)hipFree(op->a2);
+  (void)hipFree(op->ccs);
+  op->a1 = a1;
+  op->a2 = a2;
+  op->ccs = ccs;
+  op->cap_rhs = nrhs;
+  return DDA_OK;
+}
+
+// Uploads sqrt(cc) for RHS first .. first+nrhs-1 ([nrhs][nmat][3] complex double,
+// m^(3/2)), d^(-3/2) scaled into the operator's precision.
+static dda_status upload_ccs(dda_op_s* op, const double* cc_sqrt, int first, int nrhs) {
+  const double f = std::pow(op->grid_unit, -1.5);
+  const size_t n = 3ull * op->nmat * nrhs;
+  const size_t off = 3ull * op->nmat * first;
+  hipError_t e;
+  if (op->prec == 64) {
+    std::vector<double2> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_double2(cc_sqrt[2 * i] * f, cc_sqrt[2 * i + 1] * f);
+    e = hipMemcpyAsync((double2*)op->ccs + off, h.data(), n * sizeof(double2), hipMemcpyHostToDevice, op->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
+  } else {
+    std::vector<float2> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_float2((float)(cc_sqrt[2 * i] * f), (float)(cc_sqrt[2 * i + 1] * f));
+    e = hipMemcpyAsync((float2*)op->ccs + off, h.data(), n * sizeof(float2), hipMemcpyHostToDevice, op->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
+  }
+  DDA_TRY_HIP(e);
+  return DDA_OK;
+}
+
 extern "C" dda_status dda_op_set_cc_sqrt(dda_op_t op, const double* cc_sqrt) {
   if (!op || !cc_sqrt) {
     set_error("dda_op_set_cc_sqrt: null argument");
     return DDA_INVALID_ARG;
   }
-  const double f = std::pow(op->grid_unit, -1.5);
-  const size_t n = 3ull * op->nmat;
-  op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 2 * n);
-  hipError_t e;
-  if (op->prec == 64) {
-    std::vector<double2> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = make_double2(cc_sqrt[2 * i] * f, cc_sqrt[2 * i + 1] * f);
-    e = hipMemcpyAsync(op->ccs, h.data(), n * sizeof(double2), hipMemcpyHostToDevice, op->ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
-  } else {
-    std::vector<float2> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = make_float2((float)(cc_sqrt[2 * i] * f), (float)(cc_sqrt[2 * i + 1] * f));
-    e = hipMemcpyAsync(op->ccs, h.data(), n * sizeof(float2), hipMemcpyHostToDevice, op->ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
-  }
-  DDA_TRY_HIP(e);
+  op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 6ull * op->nmat);
+  DDA_TRY(upload_ccs(op, cc_sqrt, 0, 1));
   op->cc_set = true;
+  op->cc_rhs = 1;  // a batch call must set the others again
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs) {
+  if (!op || !cc_sqrt || nrhs < 1) {
+    set_error("dda_op_set_cc_sqrt_batch: null argument or nrhs < 1");
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY(ensure_rhs_capacity(op, nrhs));
+  DDA_TRY(upload_ccs(op, cc_sqrt, 0, nrhs));
+  op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 6ull * op->nmat * nrhs);
+  op->cc_set = true;
+  op->cc_rhs = nrhs;
   return DDA_OK;
 }
 
 extern "C" dda_status dda_op_apply(dda_op_t op, const void* x_dev, void* y_dev, int conj, dda_stream_t stream) {
-  if (!op || !x_dev || !y_dev || x_dev == y_dev) {
-    set_error("dda_op_apply: null or aliasing vectors");
+  return dda_op_apply_batch(op, x_dev, y_dev, 1, conj, stream);
+}
+
+extern "C" dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj,
+                                         dda_stream_t stream) {
+  if (!op || !x_dev || !y_dev || x_dev == y_dev || nrhs < 1) {
+    set_error("dda_op_apply: null or aliasing vectors, or nrhs < 1");
     return DDA_INVALID_ARG;
   }
-  if (!op->cc_set) {
-    set_error("dda_op_apply: dda_op_set_cc_sqrt has not been called");
+  if (!op->cc_set || nrhs > op->cc_rhs) {
+    set_error("dda_op_apply: couple constants not set for every right-hand side");
     return DDA_INVALID_ARG;
   }
   hipStream_t st = stream ? (hipStream_t)stream : op->ctx->stream;
-  if (op->prec == 64) return launch_matvec<double>(op, x_dev, y_dev, conj, st, nullptr);
-  return launch_matvec<float>(op, x_dev, y_dev, conj, st, nullptr);
+  if (op->prec == 64) return launch_matvec<double>(op, x_dev, y_dev, conj, st, nullptr, nrhs, 0);
+  return launch_matvec<float>(op, x_dev, y_dev, conj, st, nullptr, nrhs, 0);
 }
 
 extern "C" dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj) {

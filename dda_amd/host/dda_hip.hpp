@@ -134,8 +134,25 @@ class GpuMatVec {
     check(dda_op_set_cc_sqrt(h_, reinterpret_cast<const double*>(ccSqrt.data())), "setCoupleConstants");
     cc_ = ccSqrt;
   }
+  // One set per right-hand side of a block matvec / batched solve (no
+  // counterpart in the reference, which handles one incident field at a time).
+  void setCoupleConstantsBatch(const std::vector<std::vector<std::array<std::complex<double>, 3>>>& ccSqrt) {
+    std::vector<std::array<std::complex<double>, 3>> flat;
+    for (const auto& c : ccSqrt) {
+      if (c.size() != p_.matCount) throw Error(DDA_INVALID_ARG, "setCoupleConstantsBatch: one entry per material");
+      flat.insert(flat.end(), c.begin(), c.end());
+    }
+    check(dda_op_set_cc_sqrt_batch(h_, reinterpret_cast<const double*>(flat.data()), (int)ccSqrt.size()),
+          "setCoupleConstantsBatch");
+  }
   void apply(const DeviceVector<T>& arg, DeviceVector<T>& result, bool conj, void* stream = nullptr) {
     check(dda_op_apply(h_, arg.data(), result.data(), conj ? 1 : 0, stream), "GpuMatVec::apply");
+  }
+  // arg/result hold nrhs vectors of vecSize() back to back.
+  void applyBatch(const DeviceVector<T>& arg, DeviceVector<T>& result, int nrhs, bool conj, void* stream = nullptr) {
+    if (arg.size() < (size_t)nrhs * p_.vecSize() || result.size() < (size_t)nrhs * p_.vecSize())
+      throw Error(DDA_INVALID_ARG, "GpuMatVec::applyBatch: vector size");
+    check(dda_op_apply_batch(h_, arg.data(), result.data(), nrhs, conj ? 1 : 0, stream), "GpuMatVec::applyBatch");
   }
   const Problem& ddaParams() const { return p_; }
   const std::array<uint32_t, 3>& grid() const { return grid_; }
@@ -201,7 +218,34 @@ class IterativeSolverBase {
     check(s, "getPolVec");
     return out;
   }
+  // Batched getPolVec: every field of `eincs` solved in lockstep, each result
+  // bitwise the one getPolVec gives it alone. Per-field outcomes are in
+  // lastInfos() (status per field); throws only for errors of the call itself.
+  std::vector<std::shared_ptr<std::vector<std::complex<double>>>> getPolVecs(
+      const std::vector<std::vector<std::complex<double>>>& eincs, double eps, std::ostream& log) {
+    const size_t n = op_.ddaParams().vecSize();
+    std::vector<std::complex<double>> flat;
+    flat.reserve(n * eincs.size());
+    for (const auto& e : eincs) {
+      if (e.size() != n) throw Error(DDA_INVALID_ARG, "getPolVecs: vector size");
+      flat.insert(flat.end(), e.begin(), e.end());
+    }
+    std::vector<std::complex<double>> out(flat.size());
+    infos_.assign(eincs.size(), dda_solve_info{});
+    dda_status s = dda_solver_solve_batch(h_, reinterpret_cast<const double*>(flat.data()), (int)eincs.size(), eps,
+                                          nullptr, reinterpret_cast<double*>(out.data()), infos_.data());
+    if (s == DDA_HIP_ERROR || s == DDA_OUT_OF_MEMORY || s == DDA_NOT_SUPPORTED) check(s, "getPolVecs");
+    std::vector<std::shared_ptr<std::vector<std::complex<double>>>> res;
+    for (size_t r = 0; r < eincs.size(); r++) {
+      res.push_back(std::make_shared<std::vector<std::complex<double>>>(out.begin() + r * n, out.begin() + (r + 1) * n));
+      log << "rhs " << r << ": iterations = " << infos_[r].iterations << ", |r|/|b| = " << infos_[r].rel_residual
+          << ", status = " << infos_[r].status << std::endl;
+    }
+    if (!infos_.empty()) info_ = infos_[0];
+    return res;
+  }
   const dda_solve_info& lastInfo() const { return info_; }
+  const std::vector<dda_solve_info>& lastInfos() const { return infos_; }
   std::vector<double> history() const {
     std::vector<double> h(info_.iterations + 1);
     uint64_t w = 0;
@@ -218,6 +262,7 @@ class IterativeSolverBase {
   GpuMatVec<double>& op_;
   dda_solver_t h_ = nullptr;
   dda_solve_info info_{};
+  std::vector<dda_solve_info> infos_;
 };
 
 // new GpuQmrCs<T>(pool, queues, ddaParams, matVec, maxIter, accounting) etc.

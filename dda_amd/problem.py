@@ -170,6 +170,31 @@ def einc_plane(params, prop, pol):
     return params.to_soa(ph[:, None] * np.asarray(pol, np.float64)[None, :])
 
 
+def directions(n, seed):
+    """n propagation directions uniform on the unit sphere (z uniform in [-1, 1],
+    azimuth uniform in [0, 2 pi)), reproducible from the seed."""
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2 * math.pi, n)
+    t = np.sqrt(1 - z * z)
+    return np.stack([t * np.cos(phi), t * np.sin(phi), z], axis=1)
+
+
+def incident_batch(params, dirs, kind="ldr"):
+    """Right-hand sides of an orientation batch: for each direction both
+    polarisations of Beam.cpp:34-51, in that order. Returns (einc [2n, vec_size],
+    cc_sqrt [2n, nmat, 3], props [2n, 3], pols [2n, 3])."""
+    eincs, ccs, props, pols = [], [], [], []
+    for d in dirs:
+        prop, pol1, pol2 = beam_pols(d)
+        for pol in (pol1, pol2):
+            eincs.append(einc_plane(params, prop, pol))
+            ccs.append(couple_constants(params, prop, pol, kind)[1])
+            props.append(prop)
+            pols.append(pol)
+    return np.array(eincs), np.array(ccs), np.array(props), np.array(pols)
+
+
 def ext_cross(params, pvec, einc):
     """Cext = 4 pi k sum Im(p_i . conj(E_i)) (Beam.cpp:58-68)."""
     p = params.from_soa(pvec)

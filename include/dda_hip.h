@@ -107,6 +107,16 @@ dda_status dda_op_set_cc_sqrt(dda_op_t op, const double* cc_sqrt);
  * vectors of the operator's precision, enqueued on `stream`. x and y must not
  * alias. */
 dda_status dda_op_apply(dda_op_t op, const void* x_dev, void* y_dev, int conj, dda_stream_t stream);
+/* Block matvec (no counterpart in the reference, which solves one incident
+ * field at a time): nrhs right-hand sides that share the geometry and the
+ * interaction tensor, each with its own couple constants (LDR makes them
+ * depend on the propagation direction and polarisation,
+ * DDA/PolarizabilityDescription.cpp:96-104). Vectors are [nrhs][3][vec_stride]
+ * back to back. dda_op_set_cc_sqrt_batch takes [nrhs][nmat][3] complex double
+ * and sizes the operator's buffers for nrhs; each right-hand side's result is
+ * bitwise the one dda_op_apply gives it alone. */
+dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs);
+dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
 /* MatVecGpu::apply, DDA/MatVecGpu.cpp:51-68: host complex double in and out;
  * synchronous. */
 dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);
@@ -136,7 +146,9 @@ typedef struct dda_solve_info {
   uint64_t refinements;  /* outer FP64 residual evaluations (mixed mode), else 0 */
   double rel_residual;   /* sqrt(residScale * inprodR) at exit, as the reference logs it */
   double true_rel_residual; /* |b - A x| / |b| recomputed in FP64 after the solve */
-  double seconds;        /* wall time of the solve */
+  double seconds;        /* wall time of the solve (of the whole batch) */
+  int32_t status;        /* dda_status of this right-hand side */
+  int32_t reserved;
 } dda_solve_info;
 
 typedef enum dda_solver_mode {
@@ -172,7 +184,18 @@ dda_status dda_solver_destroy(dda_solver_t s);
  * p = sqrt(a) x. Stops when |r|^2 < eps^2 |b|^2. */
 dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, const double* start,
                             double* pol_out, dda_solve_info* info);
-/* residScale * |r|^2 after each iteration of the last solve (index 0 = start).
+/* A batch of nrhs incident fields ([nrhs][3][vec_stride] each for einc, start,
+ * pol_out), solved in lockstep with one block matvec per iteration and
+ * per-right-hand-side convergence masking (no counterpart in the reference,
+ * which solves one field at a time). info has nrhs entries; the return value
+ * is the first non-OK status among them. Couple constants come from
+ * dda_op_set_cc_sqrt_batch, or dda_op_set_cc_sqrt for all alike. A right-hand
+ * side's result is bitwise the one it gets when solved alone. */
+dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, double eps, const double* start,
+                                  double* pol_out, dda_solve_info* info);
+/* residScale * |r|^2 after each iteration of the last solve (index 0 = start;
+ * of right-hand side 0 after a batch; in mixed mode the FP64 |r|^2/|b|^2 after
+ * each refinement step).
  * Writes min(n, iterations + 1) values and returns that count in *written. */
 dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n, uint64_t* written);
 

@@ -65,6 +65,18 @@ int main(int argc, char** argv) {
     auto pol = solver.getPolVec(einc, fd[2], std::cout);
     uint64_t it = solver.lastInfo().iterations;
 
+    // batch of two fields sharing the couple constants: each bitwise the single solve
+    {
+      std::vector<cd> e2(einc.size());
+      for (size_t i = 0; i < einc.size(); i++) e2[i] = einc[i] * cd(0.5, -0.25);
+      auto pols = solver.getPolVecs({e2, einc}, fd[2], std::cout);
+      if (pols.size() != 2 || *pols[1] != *pol || solver.lastInfos()[1].iterations != it ||
+          solver.lastInfos()[1].status != DDA_OK) {
+        std::cerr << "batched solve differs from the single solve\n";
+        return 1;
+      }
+      std::cout << "batch ok" << std::endl;
+    }
     std::ofstream o(argv[2], std::ios::binary);
     o.write(reinterpret_cast<const char*>(y64.data()), sizeof(cd) * y64.size());
     o.write(reinterpret_cast<const char*>(y32.data()), sizeof(cd) * y32.size());

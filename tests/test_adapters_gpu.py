@@ -34,7 +34,7 @@ def test_cpp_adapters_match_oracle(tmp_path):
     out = tmp_path / "out.bin"
     r = subprocess.run([BIN, str(case), str(out)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
-    assert r.returncode == 0 and "adapter_check ok" in r.stdout
+    assert r.returncode == 0 and "adapter_check ok" in r.stdout and "batch ok" in r.stdout
     n = params.vec_size
     data = np.fromfile(out, np.complex128, 3 * n)
     it = int(np.fromfile(out, np.uint64, offset=3 * n * 16)[0])
