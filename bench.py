@@ -31,8 +31,9 @@ PASS_NAMES = ["k_pass1 (X-FFT + scatter)", "k_colfft (Y-FFT)", "k_pass3 (Z-FFT, 
               "k_colfft (inverse Y)", "k_pass5 (inverse X + gather)"]
 
 
-def pass_bytes(nv, S, box, grid, n_rows_nonempty, c):
-    """Algorithmic HBM bytes of each of the five kernels for one matvec."""
+def pass_bytes(nv, S, box, grid, n_rows_nonempty, c, nrhs=1):
+    """Algorithmic HBM bytes of each of the five kernels for one matvec over
+    nrhs right-hand sides (the octant tensor is read once per launch)."""
     bx, by, bz = box
     gx, gy, gz = grid
     nrows = by * bz
@@ -41,26 +42,27 @@ def pass_bytes(nv, S, box, grid, n_rows_nonempty, c):
     a1 = 3 * bz * by * gx * c
     a2 = 3 * bz * gy * gx * c
     return [
-        3 * nv * c + idx + nv + a1,                      # x, index, materials -> A1
-        a1 + a2,                                         # A1 -> A2
-        2 * a2 + oct_d,                                  # A2 (+ octant D) -> A2
-        a2 + a1,                                         # A2 -> A1
-        3 * n_rows_nonempty * gx * c + 3 * nv * c + 3 * S * c + idx + nv,  # A1, x -> y
+        nrhs * (3 * nv * c + a1) + idx + nv,             # x, index, materials -> A1
+        nrhs * (a1 + a2),                                # A1 -> A2
+        nrhs * 2 * a2 + oct_d,                           # A2 (+ octant D) -> A2
+        nrhs * (a2 + a1),                                # A2 -> A1
+        nrhs * (3 * n_rows_nonempty * gx * c + 3 * nv * c + 3 * S * c) + idx + nv,  # A1, x -> y
     ]
 
 
-def b_mv(nv, box, grid, c):
+def b_mv(nv, box, grid, c, nrhs=1):
+    """B_mv of one matvec; for a block of nrhs the tensor term counts once."""
     bx, by, bz = box
     gx, gy, gz = grid
-    return c * (9 * nv + 12 * gx * by * bz + 12 * gx * by * gz + 6 * gx * gy * gz)
+    return c * (nrhs * (9 * nv + 12 * gx * by * bz + 12 * gx * by * gz) + 6 * gx * gy * gz)
 
 
-def measured_traffic(diameter, grid, prec, pass_index):
+def measured_traffic(diameter, grid, prec, pass_index, nrhs=1):
     """HBM bytes per launch of one matvec kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size),
     or None when no measurement exists for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic_matvec_sphere128.md")
-    if diameter != 128 or tuple(grid) != (256, 256, 256) or not os.path.exists(path):
+    if nrhs != 1 or diameter != 128 or tuple(grid) != (256, 256, 256) or not os.path.exists(path):
         return None
     text = open(path).read()
     blob = text.split("```json", 1)[1].split("```", 1)[0]
@@ -110,8 +112,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--runs", default="fp64:qmr,mixed:qmr,mixed:bicgstab",
-                    help="mode:solver pairs to time; value = the last one, the FP64 QMR-CS run is the baseline")
+    ap.add_argument("--runs", default="fp64:qmr,fp64:bicgstab,mixed:qmr,mixed:bicgstab,mixed:qmr:batch,"
+                                      "mixed:bicgstab:batch",
+                    help="mode:solver[:batch] runs to time (batch = both polarisations in one block solve); "
+                         "value = the last one, the FP64 QMR-CS run is the baseline")
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -151,11 +155,21 @@ def main():
         return matvec_profile(ctx, op, g, params, rhs, args.matvec_profile, D, P, np)
 
     def run_mode(run):
-        mode, kind = run.split(":")
+        parts = run.split(":")
+        mode, kind, batch = parts[0], parts[1], len(parts) > 2 and parts[2] == "batch"
         solver = D.SOLVER_CLASSES[kind](op, max_iter=20000, mixed=(mode == "mixed"))
 
         def step():
             out = []
+            if batch:
+                op.set_couple_constants_batch(np.array([r[2] for r in rhs]))
+                ps, infos = solver.get_pol_vecs(np.array([r[3] for r in rhs]), args.eps)
+                bad = [i for i in infos if i["status"] != D.OK]
+                if bad:
+                    raise RuntimeError("batched solve failed: %s" % bad)
+                for (pol, cc, cs, einc), p, inf in zip(rhs, ps, infos):
+                    out.append((p, inf, cc, einc, pol))
+                return out
             for pol, cc, cs, einc in rhs:
                 solver.set_couple_constants(cs)
                 p = solver.get_pol_vec(einc, args.eps)
@@ -183,19 +197,21 @@ def main():
     for mode in modes:
         runs[mode] = run_mode(mode)
     final_run = modes[-1]
-    final_mode, final_solver = final_run.split(":")
+    final_mode, final_solver = final_run.split(":")[:2]
+    final_batch = final_run.endswith(":batch")
     results, elapsed, _ = runs[final_run]
     ms, cnt, skipped = runs[final_run][2]
 
     nsolves = 2 * args.steps
     value = nsolves / elapsed
     c = 8 if final_mode == "mixed" else 16
-    pb = pass_bytes(nv, S, box, grid, rows, c)
+    kb = len(rhs) if final_batch else 1  # right-hand sides per matvec launch
+    pb = pass_bytes(nv, S, box, grid, rows, c, kb)
     avg_ms = [ms[i] / cnt[i] if cnt[i] else float("nan") for i in range(5)]
     dom = int(np.nanargmax(ms))
     achieved = pb[dom] / (avg_ms[dom] * 1e-3) / 1e9
     mv_ms = float(np.nansum(avg_ms))
-    bmv = b_mv(nv, box, grid, c)
+    bmv = b_mv(nv, box, grid, c, kb)
 
     # observables of the last step, checked against Mie theory (volume-equivalent sphere)
     a_eq = (3 * nv / (4 * math.pi)) ** (1 / 3) * d
@@ -215,11 +231,13 @@ def main():
         o = observe(res)
         pms, pcnt, _ = prof
         cm = 8 if mode.startswith("mixed") else 16
-        pbm = pass_bytes(nv, S, box, grid, rows, cm)
+        km = len(rhs) if mode.endswith(":batch") else 1
+        pbm = pass_bytes(nv, S, box, grid, rows, cm, km)
         pav = [pms[i] / pcnt[i] if pcnt[i] else float("nan") for i in range(5)]
         modes_out[mode] = {"solves_per_s": nsolves / el, "ms_per_step": 1e3 * el / args.steps,
                            "matvec_ms": float(np.nansum(pav)),
-                           "matvec_gbs_vs_bmv": b_mv(nv, box, grid, cm) / (float(np.nansum(pav)) * 1e-3) / 1e9,
+                           "rhs_per_matvec": km,
+                           "matvec_gbs_vs_bmv": b_mv(nv, box, grid, cm, km) / (float(np.nansum(pav)) * 1e-3) / 1e9,
                            "pass_avg_us": [1e3 * v for v in pav],
                            "pass_gbs": [pbm[i] / (pav[i] * 1e-3) / 1e9 for i in range(5)],
                            "iterations": [x["iterations"] for x in o],
@@ -229,7 +247,7 @@ def main():
     if "fp64:qmr" in modes_out:
         f = modes_out["fp64:qmr"]
         for k in list(modes_out):
-            if k.startswith("mixed"):
+            if k != "fp64:qmr":
                 m_ = modes_out[k]
                 m_["vs_fp64_qmr"] = {
                     "speedup": m_["solves_per_s"] / f["solves_per_s"],
@@ -281,15 +299,17 @@ def main():
         "dtype": "f32+f64" if final_mode == "mixed" else "f64",
         "data": "synthetic (generated sphere geometry, plane-wave incident field)",
         "config": {"workload": "sphere %d dipoles across (nv=%d), m=1.5+0.1i, |m|kd=0.5, grid %s, prop z, "
-                               "2 polarisations, %s to resid %g, %s" % (args.diameter, nv, "x".join(map(str, grid)),
-                                                                      final_solver, args.eps,
-                                                                      "FP32 Krylov in FP64 refinement"
-                                                                      if final_mode == "mixed" else "FP64"),
-                   "nv": nv, "grid": list(grid), "solver": final_solver, "mode": final_mode, "eps": args.eps,
+                               "2 polarisations%s, %s to resid %g, %s" % (
+                                   args.diameter, nv, "x".join(map(str, grid)),
+                                   " solved as one 2-RHS block" if final_batch else " solved one after the other",
+                                   final_solver, args.eps,
+                                   "FP32 Krylov in FP64 refinement" if final_mode == "mixed" else "FP64"),
+                   "nv": nv, "grid": list(grid), "solver": final_solver, "mode": final_mode, "batch": final_batch,
+                   "eps": args.eps,
                    "baseline": "fp64:qmr (the reference's default solver, all FP64)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.diameter, grid, "fp32" if c == 8 else "fp64", dom),
+                     "traffic": measured_traffic(args.diameter, grid, "fp32" if c == 8 else "fp64", dom, kb),
                      "traffic_source": "profiles/traffic_matvec_sphere128.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                                        "calibrated)",
                      "kernel": PASS_NAMES[dom],
@@ -297,7 +317,7 @@ def main():
                      "launches": int(cnt[dom]), "noop_launches_excluded": skipped,
                      "measured_in": "%s timed region (%s kernels)" % (final_mode,
                                                                        "FP32" if c == 8 else "FP64")},
-        "matvec": {"ms": mv_ms, "bmv_bytes": bmv, "gbs_vs_bmv": bmv / (mv_ms * 1e-3) / 1e9,
+        "matvec": {"ms": mv_ms, "rhs_per_launch": kb, "bmv_bytes": bmv, "gbs_vs_bmv": bmv / (mv_ms * 1e-3) / 1e9,
                    "frac_vs_bmv": bmv / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "pass_avg_us": [1e3 * v for v in avg_ms], "pass_bytes": pb,
                    "pass_gbs": [pb[i] / (avg_ms[i] * 1e-3) / 1e9 for i in range(5)]},
