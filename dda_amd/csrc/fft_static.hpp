@@ -79,6 +79,22 @@ template <int R> struct SmallFactor {
 
 template <int R, bool INV, typename C> struct RDft;
 
+// a * W_R^k (W_R = exp(-2 pi i / R), conjugated when INV). k and R are
+// constants once the calling loops are unrolled, so the branches fold away and
+// multiples of a quarter turn become swaps and sign changes.
+template <bool INV, typename C>
+__device__ __forceinline__ C cmul_w(C a, int k, int R) {
+  typedef decltype(C().x) F;
+  k = ((k % R) + R) % R;
+  if (k == 0) return a;
+  if (2 * k == R) return cmake<C>(-a.x, -a.y);
+  if (4 * k == R) return INV ? cmul_pi(a) : cmul_mi(a);
+  if (4 * k == 3 * R) return INV ? cmul_mi(a) : cmul_pi(a);
+  const F c = (F)cexpr_trig(k, R, 0);
+  const F s = (F)cexpr_trig(k, R, 1);
+  return cmul(a, cmake<C>(c, INV ? s : -s));
+}
+
 template <bool INV, typename C> struct RDft<1, INV, C> {
   static __device__ inline void run(C*) {}
 };
@@ -148,7 +164,6 @@ template <int R, bool INV, typename C> struct RDft {
   static_assert(R == 6 || R == 8 || R == 10 || R == 12 || R == 16 || R == 20 || R == 24 || R == 32,
                 "unsupported in-register DFT length");
   static __device__ inline void run(C* a) {
-    typedef decltype(C().x) F;
     constexpr int P = SmallFactor<R>::value;
     constexpr int Q = R / P;
     C b[R];
@@ -159,15 +174,7 @@ template <int R, bool INV, typename C> struct RDft {
       for (int p = 0; p < P; p++) t[p] = a[q + Q * p];
       RDft<P, INV, C>::run(t);
 #pragma unroll
-      for (int k1 = 0; k1 < P; k1++) {
-        if (q * k1 != 0) {
-          const F c = (F)cexpr_trig(q * k1, R, 0);
-          const F s = (F)cexpr_trig(q * k1, R, 1);
-          const C w = cmake<C>(c, INV ? s : -s);
-          t[k1] = cmul(t[k1], w);
-        }
-        b[q + Q * k1] = t[k1];
-      }
+      for (int k1 = 0; k1 < P; k1++) b[q + Q * k1] = cmul_w<INV>(t[k1], q * k1, R);
     }
 #pragma unroll
     for (int k1 = 0; k1 < P; k1++) {
@@ -177,6 +184,28 @@ template <int R, bool INV, typename C> struct RDft {
       RDft<Q, INV, C>::run(t);
 #pragma unroll
       for (int k2 = 0; k2 < Q; k2++) a[k1 + P * k2] = t[k2];
+    }
+  }
+};
+
+// DFT of length R (even) whose inputs a[R/2 .. R-1] are zero; those slots are
+// not read. X[2j] = DFT_{R/2}(a)[j], X[2j+1] = DFT_{R/2}(a[m] W_R^m)[j].
+template <int R, bool INV, typename C> struct RDftHalfIn {
+  static_assert(R % 2 == 0, "half-input DFT needs an even length");
+  static __device__ inline void run(C* a) {
+    constexpr int H = R / 2;
+    C e[H], o[H];
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+      e[m] = a[m];
+      o[m] = cmul_w<INV>(a[m], m, R);
+    }
+    RDft<H, INV, C>::run(e);
+    RDft<H, INV, C>::run(o);
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+      a[2 * j] = e[j];
+      a[2 * j + 1] = o[j];
     }
   }
 };
@@ -220,8 +249,11 @@ template <int R, int... Rs> struct Reverse<RadixList<R, Rs...>> {
   typedef typename Append<typename Reverse<RadixList<Rs...>>::type, R>::type type;
 };
 
-// Stage helpers. v[i * R + m] is input m of butterfly j_i = t + i * TPC.
-template <int N, int PPT, int R, int NS, bool INV, typename C>
+// Stage helpers. v[i * R + m] is input m of butterfly j_i = t + i * TPC. The
+// twiddle W_{NS R}^k comes from the table of N roots of unity, its powers by
+// repeated multiplication. HALF (first stage only): inputs m >= R/2 are zero
+// and not read.
+template <int N, int PPT, int R, int NS, bool INV, bool HALF, typename C>
 __device__ inline void stage_compute(C* v, int t, const C* __restrict__ tw) {
   constexpr int TPC = N / PPT;
 #pragma unroll
@@ -238,7 +270,10 @@ __device__ inline void stage_compute(C* v, int t, const C* __restrict__ tw) {
         if (m + 1 < R) w = cmul(w, w1);
       }
     }
-    RDft<R, INV, C>::run(a);
+    if (HALF && NS == 1)
+      RDftHalfIn<R, INV, C>::run(a);
+    else
+      RDft<R, INV, C>::run(a);
   }
 }
 
@@ -271,25 +306,27 @@ __device__ inline void stage_load(C* v, int t, const C* tile, int lp, int l) {
 // Runs all stages of RadixList from registers to registers with LDS exchanges
 // in between. On entry v holds the first stage's inputs, on exit the last
 // stage's outputs (both in the layout described above). Ends with a barrier if
-// any exchange happened, so the caller may reuse the tile.
-template <int N, int PPT, bool INV, int NS, typename C, typename L> struct RunStages;
+// any exchange happened, so the caller may reuse the tile. HALF: the upper half
+// of every first-stage butterfly's inputs is zero (a zero-padded transform of
+// at most N/2 points) and is neither read nor computed with.
+template <int N, int PPT, bool INV, int NS, typename C, typename L, bool HALF = false> struct RunStages;
 
-template <int N, int PPT, bool INV, int NS, typename C, int R>
-struct RunStages<N, PPT, INV, NS, C, RadixList<R>> {
+template <int N, int PPT, bool INV, int NS, typename C, int R, bool HALF>
+struct RunStages<N, PPT, INV, NS, C, RadixList<R>, HALF> {
   static __device__ inline void run(C* v, int t, C* tile, int lp, int l, const C* tw) {
-    stage_compute<N, PPT, R, NS, INV>(v, t, tw);
+    stage_compute<N, PPT, R, NS, INV, HALF>(v, t, tw);
   }
 };
 
-template <int N, int PPT, bool INV, int NS, typename C, int R, int R2, int... Rs>
-struct RunStages<N, PPT, INV, NS, C, RadixList<R, R2, Rs...>> {
+template <int N, int PPT, bool INV, int NS, typename C, int R, int R2, int... Rs, bool HALF>
+struct RunStages<N, PPT, INV, NS, C, RadixList<R, R2, Rs...>, HALF> {
   static __device__ inline void run(C* v, int t, C* tile, int lp, int l, const C* tw) {
-    stage_compute<N, PPT, R, NS, INV>(v, t, tw);
+    stage_compute<N, PPT, R, NS, INV, HALF>(v, t, tw);
     stage_store<N, PPT, R, NS>(v, t, tile, lp, l);
     __syncthreads();
     stage_load<N, PPT, R2>(v, t, tile, lp, l);
     __syncthreads();
-    RunStages<N, PPT, INV, NS * R, C, RadixList<R2, Rs...>>::run(v, t, tile, lp, l, tw);
+    RunStages<N, PPT, INV, NS * R, C, RadixList<R2, Rs...>, false>::run(v, t, tile, lp, l, tw);
   }
 };
 
@@ -301,6 +338,14 @@ template <int N, int PPT, typename L>
 __device__ inline int first_point(int t, int e) {
   constexpr int R0 = FirstRadix<L>::value;
   return t + (e / R0) * (N / PPT) + (e % R0) * (N / R0);
+}
+
+// Whether v[e] of that layout lies in the upper half of the transform (point
+// >= N/2) for every thread: m = e mod R0 >= R0 / 2 with R0 even. A transform of
+// at most N/2 points neither reads (first stage) nor writes (last stage) them.
+template <typename L>
+__device__ __forceinline__ constexpr bool upper_half(int e) {
+  return FirstRadix<L>::value % 2 == 0 && (e % FirstRadix<L>::value) >= FirstRadix<L>::value / 2;
 }
 
 }  // namespace dda

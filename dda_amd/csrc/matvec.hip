@@ -367,10 +367,12 @@ __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const ty
 #pragma unroll
     for (int e = 0; e < PPT; e++) {
       const int z = first_point<N, PPT, F>(t, e);
-      reg[c][e] = (colok && z < d.Bz) ? base[c * cstride + z * zstride] : czero<C>();
+      reg[c][e] = (!upper_half<F>(e) && colok && z < d.Bz) ? base[c * cstride + z * zstride] : czero<C>();
     }
+  // Bz <= N/2 (the grid holds at least twice the box): the forward transforms
+  // skip the zero upper half and the inverse ones never form it
 #pragma unroll
-  for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F>::run(reg[c], t, tile, lp, l, tw);
+  for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F, true>::run(reg[c], t, tile, lp, l, tw);
   const long long oct = (long long)(hx + 1) * (hy + 1) * (hz + 1);
   const int fy = ky <= hy ? ky : d.Gy - ky;
   const R sy = ky <= hy ? (R)1 : (R)-1;
@@ -398,13 +400,15 @@ __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const ty
 #pragma unroll
     for (int e = 0; e < PPT; e++) {
       const int z = first_point<N, PPT, F>(t, e);
-      if (colok && z < d.Bz) base[c * cstride + z * zstride] = reg[c][e];
+      if (!upper_half<F>(e) && colok && z < d.Bz) base[c * cstride + z * zstride] = reg[c][e];
     }
   }
 }
 
 // Column FFT with a static plan (pass 2 / pass 4 and the tensor generation).
-template <typename R, int N, int T, bool INV>
+// HIN: in_len <= N/2, the zero upper half of the input is skipped; HOUT:
+// out_len <= N/2, the upper half of the output is never formed.
+template <typename R, int N, int T, bool INV, bool HIN, bool HOUT>
 __global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
                                                   const typename CxT<R>::T* __restrict__ tw, RhsFlags fl) {
   typedef typename CxT<R>::T C;
@@ -425,13 +429,13 @@ __global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, 
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const int p = first_point<N, PPT, F>(t, e);
-    v[e] = (colok && p < cp.in_len) ? src[p * cp.pstride_in] : czero<C>();
+    v[e] = (!(HIN && upper_half<F>(e)) && colok && p < cp.in_len) ? src[p * cp.pstride_in] : czero<C>();
   }
-  RunStages<N, PPT, INV, 1, C, F>::run(v, t, tile, lp, l, tw);
+  RunStages<N, PPT, INV, 1, C, F, HIN>::run(v, t, tile, lp, l, tw);
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const int p = first_point<N, PPT, Out>(t, e);
-    if (colok && p < cp.out_len) dst[p * cp.pstride_out] = v[e];
+    if (!(HOUT && upper_half<Out>(e)) && colok && p < cp.out_len) dst[p * cp.pstride_out] = v[e];
   }
 }
 
@@ -658,7 +662,11 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
   constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const size_t lds = (size_t)N * lp * sizeof(C);
-  auto k = k_colfft_s<R, N, T, INV>;
+  // the matvec's forward pass reads at most N/2 points and its inverse pass
+  // keeps at most N/2; the tensor generation transforms full lengths
+  const bool hin = !INV && 2 * cp.in_len <= N, hout = INV && 2 * cp.out_len <= N;
+  auto k = hin ? k_colfft_s<R, N, T, INV, !INV, false>
+               : hout ? k_colfft_s<R, N, T, INV, false, INV> : k_colfft_s<R, N, T, INV, false, false>;
   DDA_TRY(set_lds(k, lds));
   hipLaunchKernelGGL(k, dim3((cp.ncols + T - 1) / T, nbatch), dim3(threads), lds, st, (const C*)in, (C*)out, cp,
                      (const C*)tw, done);
