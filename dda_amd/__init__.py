@@ -34,7 +34,7 @@ EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_set_cc_sqrt_batch",
-    "dda_op_apply", "dda_op_apply_batch",
+    "dda_op_apply", "dda_op_apply_batch", "dda_op_tune",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
@@ -101,6 +101,7 @@ def lib():
         L.dda_op_set_cc_sqrt_batch.argtypes = [vp, dp, i32]
         L.dda_op_apply.argtypes = [vp, vp, vp, i32, vp]
         L.dda_op_apply_batch.argtypes = [vp, vp, vp, i32, i32, vp]
+        L.dda_op_tune.argtypes = [vp, i32, ctypes.c_char_p, sz]
         L.dda_op_apply_host.argtypes = [vp, dp, dp, i32]
         L.dda_op_get_dmatrix.argtypes = [vp, dp]
         L.dda_op_profile.argtypes = [vp, i32]
@@ -310,6 +311,12 @@ class GpuMatVec:
         xp = x.ptr if isinstance(x, DeviceVector) else x
         yp = y.ptr if isinstance(y, DeviceVector) else y
         _check(lib().dda_op_apply_batch(self.handle, xp, yp, int(nrhs), int(conj), stream), "dda_op_apply_batch")
+
+    def tune(self, nrhs=1):
+        """Autotune the kernel variants for this shape (cached per process); returns the timing summary."""
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().dda_op_tune(self.handle, int(nrhs), buf, 512), "dda_op_tune")
+        return buf.value.decode()
 
     def dmatrix(self):
         """The device tensor in the reference layout [Gx, Gz, Gy, 6] (1/m^3)."""

@@ -109,6 +109,13 @@ struct dda_op_s {
   // launch shapes
   int p1_rows = 1;   // rows per work-group in the X passes
   int col_t = 8;     // columns per work-group in the Y/Z passes
+  // Kernel variants (plan width of the static column passes: -1 default,
+  // 0 narrow, 1 wide), chosen by dda_op_tune.
+  struct Tune {
+    int col_wide[2] = {-1, -1};  // forward / inverse Y pass
+    int p3_wide = -1;            // Z pass with the tensor product
+  } tune;
+  bool tuned = false;  // dda_op_tune ran (an FP32 twin of a tuned operator is tuned too)
   int cap_rhs = 1;   // right-hand sides the grid buffers hold
   int cc_rhs = 0;    // right-hand sides with couple constants set
   bool static_z = false;  // pass 3 uses a compile-time planned kernel

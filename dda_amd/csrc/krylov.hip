@@ -1377,6 +1377,7 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
     desc.grid_unit = op->grid_unit;
     desc.wave_num = op->wave_num;
     rc = dda_op_create(op->ctx, &desc, 32, &s->op32);
+    if (rc == DDA_OK && op->tuned) rc = dda_op_tune(s->op32, 1, nullptr, 0);
   }
   if (rc == DDA_OK) rc = alloc_buffers(s, 1);
   if (rc != DDA_OK) {

@@ -21,8 +21,12 @@
 // y and z), so D(G - k) = +-D(k) per axis (DDA/DMatrixCpu.cpp:49-63).
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "fft_static.hpp"
 #include "internal.hpp"
@@ -324,12 +328,12 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
 // ------------------------------------------------------------------------------------
 
 // Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool W>
 __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
                                                 int nper, int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  typedef StaticPlan<N, sizeof(R) == 4> P;
+  typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Inv;
@@ -408,11 +412,11 @@ __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const ty
 // Column FFT with a static plan (pass 2 / pass 4 and the tensor generation).
 // HIN: in_len <= N/2, the zero upper half of the input is skipped; HOUT:
 // out_len <= N/2, the upper half of the output is never formed.
-template <typename R, int N, int T, bool INV, bool HIN, bool HOUT>
+template <typename R, int N, int T, bool W, bool INV, bool HIN, bool HOUT>
 __global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
                                                   const typename CxT<R>::T* __restrict__ tw, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  typedef StaticPlan<N, sizeof(R) == 4> P;
+  typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Out;
@@ -647,26 +651,33 @@ dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void
 
 // Columns per work-group for a static plan: 128-byte row segments, at most 256
 // threads.
-template <typename R, int N>
+template <typename R, int N, bool W>
 constexpr int static_cols() {
   constexpr int pref = sizeof(R) == 8 ? 8 : 16;
-  constexpr int tpc = N / StaticPlan<N, sizeof(R) == 4>::PPT;
+  constexpr int tpc = N / StaticPlan<N, W>::PPT;
   return pref * tpc <= 256 ? pref : 256 / tpc;
 }
 
-template <typename R, int N, bool INV>
-dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
-                             RhsFlags done) {
+// Plan width of a static kernel: 1 = wide (16 points per thread where the plan
+// has them), 0 = narrow, -1 = the precision's default (wide for complex float).
+template <typename R>
+inline bool plan_wide(int w) {
+  return w < 0 ? sizeof(R) == 4 : w != 0;
+}
+
+template <typename R, int N, bool W, bool INV>
+dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch,
+                               hipStream_t st, RhsFlags done) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N>();
-  constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
+  constexpr int T = static_cols<R, N, W>();
+  constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const size_t lds = (size_t)N * lp * sizeof(C);
   // the matvec's forward pass reads at most N/2 points and its inverse pass
   // keeps at most N/2; the tensor generation transforms full lengths
   const bool hin = !INV && 2 * cp.in_len <= N, hout = INV && 2 * cp.out_len <= N;
-  auto k = hin ? k_colfft_s<R, N, T, INV, !INV, false>
-               : hout ? k_colfft_s<R, N, T, INV, false, INV> : k_colfft_s<R, N, T, INV, false, false>;
+  auto k = hin ? k_colfft_s<R, N, T, W, INV, !INV, false>
+               : hout ? k_colfft_s<R, N, T, W, INV, false, INV> : k_colfft_s<R, N, T, W, INV, false, false>;
   DDA_TRY(set_lds(k, lds));
   hipLaunchKernelGGL(k, dim3((cp.ncols + T - 1) / T, nbatch), dim3(threads), lds, st, (const C*)in, (C*)out, cp,
                      (const C*)tw, done);
@@ -674,23 +685,36 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
   return DDA_OK;
 }
 
-template <typename R, int N>
-dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
+template <typename R, int N, bool INV>
+dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
+                             RhsFlags done, int wide) {
+  return plan_wide<R>(wide) ? launch_col_static_w<R, N, true, INV>(in, out, cp, tw, nbatch, st, done)
+                            : launch_col_static_w<R, N, false, INV>(in, out, cp, tw, nbatch, st, done);
+}
+
+template <typename R, int N, bool W>
+dda_status launch_pass3_static_w(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N>();
-  constexpr int threads = T * (N / StaticPlan<N, sizeof(R) == 4>::PPT);
+  constexpr int T = static_cols<R, N, W>();
+  constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const MvDims& d = op->dims;
   const size_t lds = (size_t)N * lp * sizeof(C);
   const int ntx = (d.Gx + T - 1) / T;
   const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
   const int nper = (nwork + 7) / 8;
-  auto k = k_pass3s<R, N, T>;
+  auto k = k_pass3s<R, N, T, W>;
   DDA_TRY(set_lds(k, lds));
   hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)op->a2, (const C*)op->dhat, d,
                      (const C*)op->tw[2], nwork, nper, nrhs, done);
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
+}
+
+template <typename R, int N>
+dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
+  return plan_wide<R>(op->tune.p3_wide) ? launch_pass3_static_w<R, N, true>(op, st, done, nrhs)
+                                        : launch_pass3_static_w<R, N, false>(op, st, done, nrhs);
 }
 
 // Lengths with a static plan; others use the run-time planned kernels.
@@ -709,11 +733,11 @@ inline bool has_static_plan(int n) {
 
 template <typename R, bool INV>
 dda_status launch_col_any(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
-                          int nbatch, hipStream_t st, RhsFlags done) {
+                          int nbatch, hipStream_t st, RhsFlags done, int wide = -1) {
   switch (p.n) {
 #define X(N) \
   case N:    \
-    return launch_col_static<R, N, INV>(in, out, cp, tw, nbatch, st, done);
+    return launch_col_static<R, N, INV>(in, out, cp, tw, nbatch, st, done, wide);
     DDA_STATIC_SIZES(X)
 #undef X
     default:
@@ -816,7 +840,8 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.By * d.Gx;
     cp.bstride_out = (long long)d.Gy * d.Gx;
-    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done)));
+    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done,
+                                      op->tune.col_wide[0])));
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
   if (op->static_z) {
@@ -868,7 +893,8 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.Gy * d.Gx;
     cp.bstride_out = (long long)d.By * d.Gx;
-    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done)));
+    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done,
+                                     op->tune.col_wide[1])));
   }
   // pass 5
   {
@@ -985,6 +1011,8 @@ static void free_op(dda_op_s* op) {
     if (b) (void)hipFree(b);
   delete op;
 }
+
+void apply_cached_tuning(dda_op_s* op);
 
 extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out) {
   if (!ctx || !desc || !out || (precision != 32 && precision != 64)) {
@@ -1149,6 +1177,7 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
       return rc;
     }
   }
+  apply_cached_tuning(op);
   *out = op;
   return DDA_OK;
 }
@@ -1346,6 +1375,138 @@ extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
           out[dst + 1] = im * sgn[c] * inv_d3;
         }
       }
+  return DDA_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Autotuning of the kernel variants
+// ------------------------------------------------------------------------------------
+
+namespace {
+
+typedef std::tuple<int, int, int, int, int, int, int> TuneKey;  // grid, box, precision
+std::mutex g_tune_mu;
+std::map<TuneKey, dda_op_s::Tune> g_tune_cache;
+
+TuneKey tune_key(const dda_op_s* op) {
+  const MvDims& d = op->dims;
+  return TuneKey(d.Gx, d.Gy, d.Gz, d.Bx, d.By, d.Bz, op->prec);
+}
+
+// Mean time of `reps` launches of one pass (after one untimed launch).
+template <typename F>
+dda_status time_pass(hipStream_t st, int reps, F&& launch, double* us) {
+  hipEvent_t e0, e1;
+  DDA_TRY_HIP(hipEventCreate(&e0));
+  DDA_TRY_HIP(hipEventCreate(&e1));
+  dda_status rc = launch();
+  if (rc == DDA_OK && hipEventRecord(e0, st) != hipSuccess) rc = DDA_HIP_ERROR;
+  for (int i = 0; i < reps && rc == DDA_OK; i++) rc = launch();
+  float ms = 0;
+  if (rc == DDA_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = DDA_HIP_ERROR;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *us = 1e3 * ms / reps;
+  return rc;
+}
+
+template <typename R>
+dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
+  const MvDims& d = op->dims;
+  hipStream_t st = op->ctx->stream;
+  const size_t csz = sizeof(typename CxT<R>::T);
+  // finite data in the grid buffers (their contents are scratch between matvecs)
+  DDA_TRY_HIP(hipMemsetAsync(op->a1, 0, 3ull * d.Bz * d.By * d.Gx * csz * nrhs, st));
+  DDA_TRY_HIP(hipMemsetAsync(op->a2, 0, 3ull * d.Bz * d.Gy * d.Gx * csz * nrhs, st));
+  const int reps = 5;
+  char buf[160];
+  dda_op_s::Tune best = op->tune;
+  for (int pass = 0; pass < 3; pass++) {
+    const bool is_static = pass == 1 ? op->static_z : has_static_plan(pass == 1 ? d.Gz : d.Gy);
+    if (!is_static) continue;
+    double t[2] = {0, 0};
+    for (int w = 0; w < 2; w++) {
+      dda_status rc;
+      if (pass == 1) {
+        op->tune.p3_wide = w;
+        rc = time_pass(st, reps, [&]() {
+          switch (d.Gz) {
+#define X(N) \
+  case N:    \
+    return launch_pass3_static<R, N>(op, st, RhsFlags(), nrhs);
+            DDA_STATIC_SIZES(X)
+#undef X
+            default:
+              return DDA_NOT_SUPPORTED;
+          }
+        }, &t[w]);
+      } else {
+        const bool inv = pass == 2;
+        ColParams cp;
+        cp.ncols = d.Gx;
+        cp.in_len = inv ? d.Gy : d.By;
+        cp.out_len = inv ? d.By : d.Gy;
+        cp.pstride_in = cp.pstride_out = d.Gx;
+        cp.bstride_in = (long long)(inv ? d.Gy : d.By) * d.Gx;
+        cp.bstride_out = (long long)(inv ? d.By : d.Gy) * d.Gx;
+        const void* in = inv ? op->a2 : op->a1;
+        void* out = inv ? op->a1 : op->a2;
+        rc = time_pass(st, reps, [&]() {
+          return inv ? launch_col_any<R, true>(in, out, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st,
+                                               RhsFlags(), w)
+                     : launch_col_any<R, false>(in, out, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st,
+                                                RhsFlags(), w);
+        }, &t[w]);
+      }
+      if (rc != DDA_OK) {
+        op->tune = best;
+        return rc;
+      }
+    }
+    const int pick = t[1] <= t[0] ? 1 : 0;
+    if (pass == 1)
+      best.p3_wide = pick;
+    else
+      best.col_wide[pass == 2] = pick;
+    std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us -> %s", report->empty() ? "" : "; ",
+                  pass == 0 ? "Y forward" : pass == 1 ? "Z + tensor" : "Y inverse", t[0], t[1],
+                  pick ? "wide" : "narrow");
+    *report += buf;
+    op->tune = best;
+  }
+  if (report->empty()) *report = "no static-plan passes to tune at this grid";
+  return DDA_OK;
+}
+
+}  // namespace
+
+// Applies a cached tuning for this operator's shape, if one exists.
+void apply_cached_tuning(dda_op_s* op) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune_cache.find(tune_key(op));
+  if (it != g_tune_cache.end()) op->tune = it->second;
+}
+
+extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len) {
+  if (!op || nrhs < 1 || nrhs > op->cap_rhs) {
+    set_error("dda_op_tune: null operator or nrhs outside 1 .. the operator's capacity");
+    return DDA_INVALID_ARG;
+  }
+  std::string rep;
+  DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  dda_status rc = op->prec == 64 ? tune_impl<double>(op, nrhs, &rep) : tune_impl<float>(op, nrhs, &rep);
+  if (rc != DDA_OK) return rc;
+  op->tuned = true;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[tune_key(op)] = op->tune;
+  }
+  if (report && report_len) {
+    std::strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
   return DDA_OK;
 }
 

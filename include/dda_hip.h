@@ -117,6 +117,15 @@ dda_status dda_op_apply(dda_op_t op, const void* x_dev, void* y_dev, int conj, d
  * bitwise the one dda_op_apply gives it alone. */
 dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs);
 dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
+/* Autotuner (no counterpart in the reference, whose FFT plans are fixed per
+ * length): times each variant of the column passes (plan width: 8 or 16
+ * points per thread) on this operator's own buffers for nrhs right-hand sides
+ * and keeps the fastest per pass. The choice is cached for the process by
+ * (grid, box, precision), so later operators of that shape start tuned, and the
+ * FP32 twin of a mixed-mode solver over a tuned operator is tuned too. Results
+ * differ between variants only by rounding. `report` (nullable) receives a
+ * one-line summary of the timings. */
+dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
 /* MatVecGpu::apply, DDA/MatVecGpu.cpp:51-68: host complex double in and out;
  * synchronous. */
 dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);
