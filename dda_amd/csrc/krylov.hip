@@ -24,6 +24,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.hpp"
@@ -821,6 +822,12 @@ struct Core {
   uint64_t hist_cap = 0;
   uint64_t max_iter = 0, max_res_increase = 0;
   C* v[8] = {};
+  // Two steady-state iterations (nr >= 2, even) captured as a HIP graph and
+  // replayed: the kernel sequence and its arguments repeat with period two
+  // (QMR-CS swaps its vector pairs every iteration). Recaptured whenever
+  // anything its launches reference changes.
+  hipGraphExec_t gexec = nullptr;
+  std::vector<const void*> gsig;
 
   dim3 g() const { return dim3(nb, nrhs); }
   dim3 b() const { return dim3(kRedThreads); }
@@ -861,7 +868,22 @@ struct Core {
     for (int i = 0; i < 8; i++) DDA_TRY_HIP(hipMalloc(&v[i], (size_t)cap * n * sizeof(C)));
     return DDA_OK;
   }
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gsig.clear();
+  }
+  std::vector<const void*> graph_signature() const {
+    std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat,
+                                  reinterpret_cast<const void*>((intptr_t)nrhs),
+                                  reinterpret_cast<const void*>((intptr_t)(op->tune.col_wide[0] * 4 +
+                                                                           op->tune.col_wide[1] * 16 +
+                                                                           op->tune.p3_wide * 64 + 1024))};
+    for (C* p : v) g.push_back(p);
+    return g;
+  }
   void release() {
+    drop_graph();
     if (dst) (void)hipFree(dst);
     if (host2) (void)hipHostFree(host2);
     if (part) (void)hipFree(part);
@@ -996,6 +1018,34 @@ struct Core {
       return DDA_OK;
     };
 
+    // The graph of two iterations: captured from the eager sequence (which
+    // leaves the vector roles as they were), then replayed.
+    const bool use_graph = !op->prof && !std::getenv("DDA_NO_GRAPHS");
+    auto launch_pair = [&]() -> dda_status {
+      std::vector<const void*> sig = graph_signature();
+      if (!gexec || sig != gsig) {
+        drop_graph();
+        hipGraph_t g = nullptr;
+        DDA_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        dda_status rc1 = enqueue_iteration();
+        if (rc1 == DDA_OK) rc1 = enqueue_iteration();
+        hipError_t e = hipStreamEndCapture(st, &g);
+        nr -= 2;
+        if (rc1 != DDA_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc1;
+        }
+        DDA_TRY_HIP(e);
+        e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DDA_TRY_HIP(e);
+        gsig = sig;
+      }
+      DDA_TRY_HIP(hipGraphLaunch(gexec, st));
+      nr += 2;
+      return DDA_OK;
+    };
+
     // Chunks of iterations are enqueued ahead of the state reads: chunk c+1 is
     // queued before the states copied after chunk c are examined, so the GPU
     // never waits on the host; the chunk length follows the slowest observed
@@ -1012,7 +1062,15 @@ struct Core {
     bool finished = false;
     dda_status rc = DDA_OK;
     while (!finished && rc == DDA_OK) {
-      for (int it = 0; it < next_len && rc == DDA_OK; it++) rc = enqueue_iteration();
+      for (int it = 0; it < next_len && rc == DDA_OK;) {
+        if (use_graph && nr >= 2 && nr % 2 == 0 && next_len - it >= 2) {
+          rc = launch_pair();
+          it += 2;
+        } else {
+          rc = enqueue_iteration();
+          it++;
+        }
+      }
       if (rc != DDA_OK) break;
       const int slot = (chunk + 1) & 1;
       DDA_TRY_HIP(hipMemcpyAsync(host2 + slot * cap, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));

@@ -85,3 +85,30 @@ def test_start_vector(ctx):
     p2 = s.get_pol_vec(einc, 1e-6, start=p)
     assert s.info["iterations"] == 0
     assert cases.rel_err(p2, p) <= 1e-12
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("kind", ["qmr", "bicgstab", "cgnr", "bicg"])
+def test_graph_replay_matches_eager(ctx, kind, mixed, monkeypatch):
+    """Steady-state iterations run as replays of a captured two-iteration HIP
+    graph; the results are bitwise those of the eager launch sequence, over
+    several solves (many replays, with eager work and synchronisation between
+    them) and after the operator's buffers grow for a batch (recapture)."""
+    print("graph", kind, "mixed", mixed, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("sphere_r7")
+    einc = P.einc_plane(params, prop, pol)
+    op = _op(ctx, g, params, cs)
+    s = D.SOLVER_CLASSES[kind](op, max_iter=5000, mixed=mixed)
+    monkeypatch.setenv("DDA_NO_GRAPHS", "1")
+    p_eager = s.get_pol_vec(einc, 1e-9)
+    it_eager = s.info["iterations"]
+    monkeypatch.delenv("DDA_NO_GRAPHS")
+    for rep in range(3):
+        p = s.get_pol_vec(einc, 1e-9)
+        assert s.info["iterations"] == it_eager and np.array_equal(p, p_eager), rep
+        D.MatVecGpu(op).apply(einc)  # eager work between the replays
+    op.set_couple_constants_batch(np.stack([cs, cs, cs]))
+    pb, info = s.get_pol_vecs(np.stack([einc, 2 * einc, einc]), 1e-9)
+    assert np.array_equal(pb[0], p_eager) and np.array_equal(pb[2], p_eager)
+    op.set_couple_constants(cs)
+    assert np.array_equal(s.get_pol_vec(einc, 1e-9), p_eager)
