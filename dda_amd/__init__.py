@@ -39,6 +39,8 @@ EXPORTS = [
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
     "dda_solver_set_inner_floor", "dda_solver_profile_read",
+    "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
+    "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
 ]
 
 
@@ -64,6 +66,26 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint64), ("matvecs", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
                 ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("cache_bytes", ctypes.c_size_t), ("solver_kind", ctypes.c_int), ("mode", ctypes.c_int),
+                ("eps", ctypes.c_double), ("max_iter", ctypes.c_uint64), ("max_pack", ctypes.c_int)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("geometry", OpDesc), ("nrhs", ctypes.c_int), ("cc_sqrt", ctypes.POINTER(ctypes.c_double)),
+                ("einc", ctypes.POINTER(ctypes.c_double))]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("status", ctypes.c_int32), ("packed_rhs", ctypes.c_int32), ("footprint", ctypes.c_size_t),
+                ("cache_hit", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -114,6 +136,15 @@ def lib():
         L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_solve_batch.argtypes = [vp, dp, i32, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_history.argtypes = [vp, dp, u64, ctypes.POINTER(u64)]
+        L.dda_dhat_cache_create.argtypes = [vp, sz, pp]
+        L.dda_dhat_cache_destroy.argtypes = [vp]
+        L.dda_dhat_cache_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(sz), ctypes.POINTER(i32)]
+        L.dda_op_create_cached.argtypes = [vp, ctypes.POINTER(OpDesc), i32, vp, pp]
+        L.dda_server_create.argtypes = [ctypes.POINTER(ServerConfig), pp]
+        L.dda_server_submit.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(u64)]
+        L.dda_server_wait.argtypes = [vp, u64, dp, ctypes.POINTER(SolveInfo), ctypes.POINTER(JobResult)]
+        L.dda_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(sz)]
+        L.dda_server_destroy.argtypes = [vp]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("dda_last_error", "dda_fft_fit"):
@@ -247,7 +278,7 @@ class GpuMatVec:
     """
 
     def __init__(self, ctx, pos, mat, box, grid_unit, wave_num, precision=64, grid=(0, 0, 0), nmat=None,
-                 vec_stride=0, dmatrix=None):
+                 vec_stride=0, dmatrix=None, cache=None):
         self.ctx = ctx
         self.pos = np.ascontiguousarray(pos, np.uint32).reshape(-1, 3)
         self.nv = self.pos.shape[0]
@@ -269,7 +300,11 @@ class GpuMatVec:
             self._dm = np.ascontiguousarray(dmatrix, np.complex128)
             d.dmatrix = _dp(self._dm.view(np.float64))
         h = ctypes.c_void_p()
-        _check(lib().dda_op_create(ctx.handle, ctypes.byref(d), precision, ctypes.byref(h)), "dda_op_create")
+        if cache is not None:
+            _check(lib().dda_op_create_cached(ctx.handle, ctypes.byref(d), precision, cache.handle, ctypes.byref(h)),
+                   "dda_op_create_cached")
+        else:
+            _check(lib().dda_op_create(ctx.handle, ctypes.byref(d), precision, ctypes.byref(h)), "dda_op_create")
         self.handle = h
         ctx._adopt(self)
         self._dm = None
@@ -470,3 +505,99 @@ class GpuBicgCs(IterativeSolver):
 
 
 SOLVER_CLASSES = {"qmr": GpuQmrCs, "bicgstab": GpuBicgStab, "cgnr": GpuCgnr, "bicg": GpuBicgCs}
+
+
+class TensorCache:
+    """Device cache of FFT'd interaction tensors keyed by (grid, kd, precision)."""
+
+    def __init__(self, ctx, max_bytes=1 << 34):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().dda_dhat_cache_create(ctx.handle, int(max_bytes), ctypes.byref(h)), "dda_dhat_cache_create")
+        self.handle = h
+
+    def stats(self):
+        c = (ctypes.c_uint64 * 3)()
+        b = ctypes.c_size_t()
+        e = ctypes.c_int()
+        _check(lib().dda_dhat_cache_stats(self.handle, c, ctypes.byref(b), ctypes.byref(e)))
+        return {"hits": c[0], "misses": c[1], "evictions": c[2], "bytes": b.value, "entries": e.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _check(lib().dda_dhat_cache_destroy(self.handle), "dda_dhat_cache_destroy")
+            self.handle = None
+
+
+class Server:
+    """Scattering-job server (dda_server_*): queue, HBM admission, `slots`
+    concurrent block solves on their own streams, tensor cache, packing of
+    jobs of the same particle."""
+
+    def __init__(self, device=0, slots=4, solver="bicgstab", mixed=True, eps=1e-6, max_iter=0, max_pack=16,
+                 hbm_budget=0, cache_bytes=0):
+        c = ServerConfig()
+        c.device, c.slots, c.hbm_budget, c.cache_bytes = device, slots, int(hbm_budget), int(cache_bytes)
+        c.solver_kind = SOLVER_KINDS[solver]
+        c.mode = MODE_MIXED if mixed else MODE_FP64
+        c.eps, c.max_iter, c.max_pack = float(eps), int(max_iter), int(max_pack)
+        h = ctypes.c_void_p()
+        _check(lib().dda_server_create(ctypes.byref(c), ctypes.byref(h)), "dda_server_create")
+        self.handle = h
+        self._shapes = {}
+
+    def submit(self, pos, mat, box, grid_unit, wave_num, einc, cc_sqrt, nmat=None, grid=(0, 0, 0)):
+        """einc [nrhs, 3*vec_stride] complex, cc_sqrt [nrhs, nmat, 3] complex -> job id."""
+        pos = np.ascontiguousarray(pos, np.uint32).reshape(-1, 3)
+        nv = pos.shape[0]
+        mat = np.ascontiguousarray(mat if mat is not None else np.zeros(nv), np.uint8)
+        nmat = int(nmat if nmat is not None else int(mat.max()) + 1)
+        e = np.ascontiguousarray(einc, np.complex128)
+        e = e.reshape(-1, e.shape[-1])
+        cs = np.ascontiguousarray(np.asarray(cc_sqrt, np.complex128).reshape(e.shape[0], nmat, 3))
+        j = JobDesc()
+        g = j.geometry
+        g.box[:] = [int(b) for b in box]
+        g.grid[:] = [int(x) for x in grid]
+        g.nv = nv
+        g.vec_stride = e.shape[1] // 3
+        g.pos = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        g.mat = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        g.nmat = nmat
+        g.grid_unit, g.wave_num = float(grid_unit), float(wave_num)
+        j.nrhs = e.shape[0]
+        j.cc_sqrt = _dp(cs.view(np.float64))
+        j.einc = _dp(e.view(np.float64))
+        jid = ctypes.c_uint64()
+        _check(lib().dda_server_submit(self.handle, ctypes.byref(j), ctypes.byref(jid)), "dda_server_submit")
+        self._shapes[jid.value] = e.shape
+        return jid.value
+
+    def wait(self, job_id):
+        """-> (pol [nrhs, 3*vec_stride], infos, result dict); status in result["status"]."""
+        shape = self._shapes.pop(job_id)
+        pol = np.zeros(shape, np.complex128)
+        infos = (SolveInfo * shape[0])()
+        res = JobResult()
+        st = lib().dda_server_wait(self.handle, job_id, _dp(pol.view(np.float64)), infos, ctypes.byref(res))
+        if st not in (OK, BREAKDOWN, MAX_ITER, STAGNATION, INVALID_ARG):
+            _check(st, "dda_server_wait")
+        return pol, [i.as_dict() for i in infos], res.as_dict()
+
+    def stats(self):
+        c = (ctypes.c_uint64 * 5)()
+        pk = ctypes.c_size_t()
+        _check(lib().dda_server_stats(self.handle, c, ctypes.byref(pk)))
+        return {"submitted": c[0], "completed": c[1], "cache_hits": c[2], "cache_misses": c[3],
+                "cache_evictions": c[4], "peak_bytes": pk.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dda_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

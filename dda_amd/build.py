@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "_lib", "libdda_hip.so")
-SOURCES = ["api.hip", "matvec.hip", "krylov.hip"]
+SOURCES = ["api.hip", "matvec.hip", "krylov.hip", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -24,7 +24,10 @@ FLAGS = [
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(SRC, src), "-o", obj]
+    if src.endswith(".cpp"):  # host-only C++
+        cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-pthread", "-c", os.path.join(SRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -51,7 +54,7 @@ def build(verbose=False):
         if out.strip() and verbose:
             print(out)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + [o for o, _ in results]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
     build_adapter_check()

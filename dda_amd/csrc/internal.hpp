@@ -55,7 +55,13 @@ struct RhsFlags {
 bool make_plan(int n, FftPlan* p);
 
 // Device twiddle table exp(-2 pi i t / n), t < n, computed in long double.
-dda_status make_twiddles(int n, int prec, void** out);
+dda_status make_twiddles(int n, int prec, void** out, hipStream_t st);
+
+// The FP32 twin of an FP64 operator: same geometry, tensor converted from the
+// FP64 one (or taken from the same cache).
+dda_status create_twin32(const struct ::dda_op_s* op, struct ::dda_op_s** out);
+// Whether this thread's last tensor-cache lookup found its entry.
+bool last_tensor_cache_hit();
 
 // Launch helpers implemented in matvec.hip.
 // nrhs right-hand sides stored back to back ([nrhs][3][S]); done (nullable)
@@ -115,7 +121,8 @@ struct dda_op_s {
     int col_wide[2] = {-1, -1};  // forward / inverse Y pass
     int p3_wide = -1;            // Z pass with the tensor product
   } tune;
-  bool tuned = false;  // dda_op_tune ran (an FP32 twin of a tuned operator is tuned too)
+  bool tuned = false;
+  struct dda_dhat_cache_s* dhat_cache = nullptr;  // set when dhat is a cache entry (not owned)  // dda_op_tune ran (an FP32 twin of a tuned operator is tuned too)
   int cap_rhs = 1;   // right-hand sides the grid buffers hold
   int cc_rhs = 0;    // right-hand sides with couple constants set
   bool static_z = false;  // pass 3 uses a compile-time planned kernel

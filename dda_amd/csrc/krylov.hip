@@ -1026,7 +1026,7 @@ struct Core {
       if (!gexec || sig != gsig) {
         drop_graph();
         hipGraph_t g = nullptr;
-        DDA_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        DDA_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         dda_status rc1 = enqueue_iteration();
         if (rc1 == DDA_OK) rc1 = enqueue_iteration();
         hipError_t e = hipStreamEndCapture(st, &g);
@@ -1101,7 +1101,8 @@ struct Core {
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     if (rc != DDA_OK) return rc;
-    DDA_TRY_HIP(hipMemcpy(fin, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost));
+    DDA_TRY_HIP(hipMemcpyAsync(fin, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));
+    DDA_TRY_HIP(hipStreamSynchronize(st));
     return DDA_OK;
   }
 };
@@ -1419,22 +1420,7 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
   s->n = 3ll * d.S;
   dda_status rc = DDA_OK;
   if (mode == DDA_MODE_MIXED) {
-    dda_op_desc desc;
-    std::memset(&desc, 0, sizeof desc);
-    desc.box[0] = d.Bx;
-    desc.box[1] = d.By;
-    desc.box[2] = d.Bz;
-    desc.grid[0] = d.Gx;
-    desc.grid[1] = d.Gy;
-    desc.grid[2] = d.Gz;
-    desc.nv = d.nv;
-    desc.vec_stride = d.S;
-    desc.pos = op->pos.data();
-    desc.mat = op->mat.data();
-    desc.nmat = op->nmat;
-    desc.grid_unit = op->grid_unit;
-    desc.wave_num = op->wave_num;
-    rc = dda_op_create(op->ctx, &desc, 32, &s->op32);
+    rc = create_twin32(op, &s->op32);
     if (rc == DDA_OK && op->tuned) rc = dda_op_tune(s->op32, 1, nullptr, 0);
   }
   if (rc == DDA_OK) rc = alloc_buffers(s, 1);
@@ -1532,7 +1518,10 @@ extern "C" dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n
     for (uint64_t i = 0; i < m; i++) out[i] = s->outer_hist[i];
   } else {
     m = std::min<uint64_t>(std::min<uint64_t>(n, s->last_iters + 1), s->c64.hist_cap);
-    if (m) DDA_TRY_HIP(hipMemcpy(out, s->c64.hist, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (m) {
+      DDA_TRY_HIP(hipMemcpyAsync(out, s->c64.hist, m * sizeof(double), hipMemcpyDeviceToHost, s->op->ctx->stream));
+      DDA_TRY_HIP(hipStreamSynchronize(s->op->ctx->stream));
+    }
   }
   if (written) *written = m;
   return DDA_OK;

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -553,7 +554,14 @@ bool make_plan(int n, FftPlan* p) {
   return true;
 }
 
-dda_status make_twiddles(int n, int prec, void** out) {
+// Host <-> device copies ordered on a stream (never on the null stream, so
+// that graph capture on another thread's stream is undisturbed), synchronous.
+static hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+dda_status make_twiddles(int n, int prec, void** out, hipStream_t st) {
   const long double pi = 3.1415926535897932384626433832795028L;
   size_t esz = prec == 64 ? sizeof(double2) : sizeof(float2);
   std::vector<char> h(esz * n);
@@ -566,7 +574,7 @@ dda_status make_twiddles(int n, int prec, void** out) {
       reinterpret_cast<float2*>(h.data())[t] = make_float2((float)c, (float)s);
   }
   DDA_TRY_HIP(hipMalloc(out, h.size()));
-  DDA_TRY_HIP(hipMemcpy(*out, h.data(), h.size(), hipMemcpyHostToDevice));
+  DDA_TRY_HIP(copy_sync(*out, h.data(), h.size(), hipMemcpyHostToDevice, st));
   return DDA_OK;
 }
 
@@ -924,7 +932,11 @@ template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hi
 // Generate the octant of the FFT'd, d^3-scaled tensor on the device in FP64:
 // fill one component on the full grid, 3-D FFT (X rows, Y and Z columns), keep
 // the octant times -1/N, converted to the operator's precision.
-static dda_status generate_dhat(dda_op_s* op) {
+// clip: offsets with |offset| >= clip per axis are zero (the reference clips at
+// the box; a tensor clipped anywhere between the box and G/2 gives the same
+// products for every dipole inside the box). Writes the octant into dst in the
+// operator's precision.
+static dda_status generate_dhat(dda_op_s* op, int cx, int cy, int cz, void* dst) {
   const MvDims& d = op->dims;
   hipStream_t st = op->ctx->stream;
   const long long total = (long long)d.Gx * d.Gy * d.Gz;
@@ -932,11 +944,11 @@ static dda_status generate_dhat(dda_op_s* op) {
   DDA_TRY_HIP(hipMalloc(&w, total * sizeof(double2)));
   void* tw64[3] = {nullptr, nullptr, nullptr};
   dda_status rc = DDA_OK;
-  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(op->plan[a].n, 64, &tw64[a]);
+  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(op->plan[a].n, 64, &tw64[a], st);
   const double kd = op->wave_num * op->grid_unit;
   const double scale = -1.0 / (double)total;
   for (int comp = 0; comp < 6 && rc == DDA_OK; comp++) {
-    hipLaunchKernelGGL(k_green_fill, dim3(2048), dim3(256), 0, st, w, comp, d.Gx, d.Gy, d.Gz, d.Bx, d.By, d.Bz, kd);
+    hipLaunchKernelGGL(k_green_fill, dim3(2048), dim3(256), 0, st, w, comp, d.Gx, d.Gy, d.Gz, cx, cy, cz, kd);
     rc = launch_row<double>(w, (long long)d.Gy * d.Gz, op->plan[0], tw64[0], st);
     if (rc != DDA_OK) break;
     ColParams cy;
@@ -954,10 +966,10 @@ static dda_status generate_dhat(dda_op_s* op) {
     rc = launch_col_any<double, false>(w, w, cz, op->plan[2], col_tile(d.Gz, 8), tw64[2], 1, st, RhsFlags());
     if (rc != DDA_OK) break;
     if (op->prec == 64)
-      hipLaunchKernelGGL(k_dhat_extract<double>, dim3(1024), dim3(256), 0, st, w, (double2*)op->dhat, comp, d.Gx, d.Gy,
+      hipLaunchKernelGGL(k_dhat_extract<double>, dim3(1024), dim3(256), 0, st, w, (double2*)dst, comp, d.Gx, d.Gy,
                          d.Gz, scale);
     else
-      hipLaunchKernelGGL(k_dhat_extract<float>, dim3(1024), dim3(256), 0, st, w, (float2*)op->dhat, comp, d.Gx, d.Gy,
+      hipLaunchKernelGGL(k_dhat_extract<float>, dim3(1024), dim3(256), 0, st, w, (float2*)dst, comp, d.Gx, d.Gy,
                          d.Gz, scale);
     if (hipGetLastError() != hipSuccess) {
       set_error("tensor generation launch failed");
@@ -973,9 +985,187 @@ static dda_status generate_dhat(dda_op_s* op) {
   return DDA_OK;
 }
 
+__global__ void k_narrow(const double2* __restrict__ in, float2* out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
 }  // namespace dda
 
 using namespace dda;
+
+// ------------------------------------------------------------------------------------
+// Device cache of FFT'd interaction tensors keyed by (grid, kd, precision)
+// ------------------------------------------------------------------------------------
+
+struct dda_dhat_cache_s {
+  struct Entry {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int refs = 0;
+    bool ready = false;
+    uint64_t last_use = 0;
+  };
+  typedef std::tuple<int, int, int, uint64_t, int> Key;  // grid, bits of kd, precision
+  int device = 0;
+  size_t max_bytes = 0, bytes = 0;
+  uint64_t clock = 0, hits = 0, misses = 0, evictions = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::map<Key, Entry> map;
+};
+
+static dda_dhat_cache_s::Key cache_key(const MvDims& d, double kd, int prec) {
+  uint64_t bits;
+  std::memcpy(&bits, &kd, sizeof bits);
+  return dda_dhat_cache_s::Key(d.Gx, d.Gy, d.Gz, bits, prec);
+}
+
+// Frees unreferenced entries, least recently used first, until `need` more
+// bytes fit under the limit (or nothing is left to free). Caller holds mu.
+static void cache_evict(dda_dhat_cache_s* c, size_t need) {
+  while (c->bytes + need > c->max_bytes) {
+    auto victim = c->map.end();
+    for (auto it = c->map.begin(); it != c->map.end(); ++it)
+      if (it->second.ready && it->second.refs == 0 &&
+          (victim == c->map.end() || it->second.last_use < victim->second.last_use))
+        victim = it;
+    if (victim == c->map.end()) return;
+    (void)hipFree(victim->second.ptr);
+    c->bytes -= victim->second.bytes;
+    c->map.erase(victim);
+    c->evictions++;
+  }
+}
+
+// The operator's tensor from the cache: a hit takes a reference; a miss builds
+// it on the operator's stream (an FP32 entry from the FP64 one when that is
+// cached, else from the Green's function clipped at G/2) while other callers of
+// the same key wait.
+static thread_local bool t_cache_hit = false;
+bool dda::last_tensor_cache_hit() { return t_cache_hit; }
+
+static dda_status cache_acquire(dda_dhat_cache_s* c, dda_op_s* op) {
+  const MvDims& d = op->dims;
+  const double kd = op->wave_num * op->grid_unit;
+  const auto key = cache_key(d, kd, op->prec);
+  const size_t bytes = 6ull * (d.Gx / 2 + 1) * (d.Gy / 2 + 1) * (d.Gz / 2 + 1) *
+                       (op->prec == 64 ? sizeof(double2) : sizeof(float2));
+  std::unique_lock<std::mutex> lk(c->mu);
+  auto it = c->map.find(key);
+  while (it != c->map.end() && !it->second.ready) {
+    c->cv.wait(lk);
+    it = c->map.find(key);
+  }
+  t_cache_hit = it != c->map.end();
+  if (it != c->map.end()) {
+    it->second.refs++;
+    it->second.last_use = ++c->clock;
+    c->hits++;
+    op->dhat = it->second.ptr;
+    op->dhat_cache = c;
+    return DDA_OK;
+  }
+  c->misses++;
+  cache_evict(c, bytes);
+  dda_dhat_cache_s::Entry& en = c->map[key];
+  en.bytes = bytes;
+  void* src64 = nullptr;
+  if (op->prec == 32) {
+    auto f = c->map.find(cache_key(d, kd, 64));
+    if (f != c->map.end() && f->second.ready) {
+      src64 = f->second.ptr;
+      f->second.refs++;  // held while converting
+    }
+  }
+  lk.unlock();
+  dda_status rc = DDA_OK;
+  hipError_t e = hipMalloc(&en.ptr, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("tensor cache: hipMalloc failed: ") + hipGetErrorString(e));
+    rc = e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+  } else if (src64) {
+    hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, op->ctx->stream, (const double2*)src64, (float2*)en.ptr,
+                       (long long)(bytes / sizeof(float2)));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("tensor cache: conversion failed: ") + hipGetErrorString(e));
+      rc = DDA_HIP_ERROR;
+    }
+  } else {
+    rc = generate_dhat(op, d.Gx / 2, d.Gy / 2, d.Gz / 2, en.ptr);
+  }
+  lk.lock();
+  if (src64) c->map.find(cache_key(d, kd, 64))->second.refs--;
+  if (rc != DDA_OK) {
+    if (en.ptr) (void)hipFree(en.ptr);
+    c->map.erase(key);
+    c->cv.notify_all();
+    return rc;
+  }
+  en.ready = true;
+  en.refs = 1;
+  en.last_use = ++c->clock;
+  c->bytes += bytes;
+  c->cv.notify_all();
+  op->dhat = en.ptr;
+  op->dhat_cache = c;
+  return DDA_OK;
+}
+
+static void cache_release(dda_dhat_cache_s* c, void* ptr) {
+  std::lock_guard<std::mutex> g(c->mu);
+  for (auto& kv : c->map)
+    if (kv.second.ptr == ptr) {
+      kv.second.refs--;
+      break;
+    }
+  cache_evict(c, 0);
+}
+
+extern "C" dda_status dda_dhat_cache_create(dda_ctx_t ctx, size_t max_bytes, dda_dhat_cache_t* out) {
+  if (!ctx || !out) {
+    set_error("dda_dhat_cache_create: null argument");
+    return DDA_INVALID_ARG;
+  }
+  dda_dhat_cache_s* c = new dda_dhat_cache_s();
+  c->device = ctx->device;
+  c->max_bytes = max_bytes;
+  *out = c;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_dhat_cache_destroy(dda_dhat_cache_t c) {
+  if (!c) return DDA_OK;
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    for (auto& kv : c->map)
+      if (kv.second.refs > 0) {
+        set_error("dda_dhat_cache_destroy: operators still use cached tensors");
+        return DDA_INVALID_ARG;
+      }
+    for (auto& kv : c->map) (void)hipFree(kv.second.ptr);
+  }
+  delete c;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_dhat_cache_stats(dda_dhat_cache_t c, uint64_t counts[3], size_t* bytes, int* entries) {
+  if (!c) {
+    set_error("dda_dhat_cache_stats: null cache");
+    return DDA_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> g(c->mu);
+  if (counts) {
+    counts[0] = c->hits;
+    counts[1] = c->misses;
+    counts[2] = c->evictions;
+  }
+  if (bytes) *bytes = c->bytes;
+  if (entries) *entries = (int)c->map.size();
+  return DDA_OK;
+}
 
 extern "C" uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot) {
   // DDA/DDAParams.cpp:45-90
@@ -1005,6 +1195,10 @@ static void free_op(dda_op_s* op) {
     (void)hipEventDestroy(r.e);
   }
   for (auto e : op->prof_pool) (void)hipEventDestroy(e);
+  if (op->dhat_cache) {
+    cache_release(op->dhat_cache, op->dhat);
+    op->dhat = nullptr;
+  }
   void* bufs[] = {op->row_start, op->row_dip, op->row_meta, op->row_list, op->mat_dev, op->ccs, op->dhat, op->a1,
                   op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y};
   for (void* b : bufs)
@@ -1014,7 +1208,10 @@ static void free_op(dda_op_s* op) {
 
 void apply_cached_tuning(dda_op_s* op);
 
-extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out) {
+// The tensor comes from desc->dmatrix, else from `cache`, else from the FP64
+// operator `narrow_from` of the same shape (converted), else it is generated.
+static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_dhat_cache_s* cache,
+                            const dda_op_s* narrow_from, dda_op_t* out) {
   if (!ctx || !desc || !out || (precision != 32 && precision != 64)) {
     set_error("dda_op_create: null argument or precision not 32/64");
     return DDA_INVALID_ARG;
@@ -1125,22 +1322,24 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
   if (rc == DDA_OK) rc = dmalloc((void**)&op->row_list, std::max<size_t>(1, row_list.size()) * 4);
   if (rc == DDA_OK) rc = dmalloc((void**)&op->mat_dev, ds.nv);
   if (rc == DDA_OK) rc = dmalloc(&op->ccs, 3ull * ds.nmat * csz);
-  if (rc == DDA_OK) rc = dmalloc(&op->dhat, dn * csz);
+  if (rc == DDA_OK && (ds.dmatrix || !cache)) rc = dmalloc(&op->dhat, dn * csz);
   if (rc == DDA_OK) rc = dmalloc(&op->a1, a1n * csz);
   if (rc == DDA_OK) rc = dmalloc(&op->a2, a2n * csz);
-  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(grid[a], precision, &op->tw[a]);
+  for (int a = 0; a < 3 && rc == DDA_OK; a++) rc = make_twiddles(grid[a], precision, &op->tw[a], ctx->stream);
   if (rc != DDA_OK) {
     free_op(op);
     return rc;
   }
   hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = hipMemcpy(op->row_start, row_start.data(), row_start.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(op->row_dip, row_dip.data(), ds.nv * 4ull, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(op->row_meta, row_meta.data(), ds.nv * 4ull, hipMemcpyHostToDevice);
+  hipStream_t cst = ctx->stream;
+  if (e == hipSuccess) e = copy_sync(op->row_start, row_start.data(), row_start.size() * 4, hipMemcpyHostToDevice, cst);
+  if (e == hipSuccess) e = copy_sync(op->row_dip, row_dip.data(), ds.nv * 4ull, hipMemcpyHostToDevice, cst);
+  if (e == hipSuccess) e = copy_sync(op->row_meta, row_meta.data(), ds.nv * 4ull, hipMemcpyHostToDevice, cst);
   if (e == hipSuccess && !row_list.empty())
-    e = hipMemcpy(op->row_list, row_list.data(), row_list.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(op->mat_dev, op->mat.data(), ds.nv, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(op->ccs, 0, 3ull * ds.nmat * csz);
+    e = copy_sync(op->row_list, row_list.data(), row_list.size() * 4, hipMemcpyHostToDevice, cst);
+  if (e == hipSuccess) e = copy_sync(op->mat_dev, op->mat.data(), ds.nv, hipMemcpyHostToDevice, cst);
+  if (e == hipSuccess) e = hipMemsetAsync(op->ccs, 0, 3ull * ds.nmat * csz, cst);
+  if (e == hipSuccess) e = hipStreamSynchronize(cst);
   if (e != hipSuccess) {
     set_error(std::string("dda_op_create: upload failed: ") + hipGetErrorString(e));
     free_op(op);
@@ -1164,14 +1363,30 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
             else
               reinterpret_cast<float2*>(h.data())[dst] = make_float2((float)re, (float)im);
           }
-    e = hipMemcpy(op->dhat, h.data(), h.size(), hipMemcpyHostToDevice);
+    e = copy_sync(op->dhat, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) {
       set_error(std::string("dda_op_create: tensor upload failed: ") + hipGetErrorString(e));
       free_op(op);
       return DDA_HIP_ERROR;
     }
+  } else if (cache) {
+    rc = cache_acquire(cache, op);
+    if (rc != DDA_OK) {
+      free_op(op);
+      return rc;
+    }
+  } else if (narrow_from && precision == 32 && narrow_from->prec == 64) {
+    hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, ctx->stream, (const double2*)narrow_from->dhat,
+                       (float2*)op->dhat, (long long)dn);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("dda_op_create: tensor conversion failed: ") + hipGetErrorString(e));
+      free_op(op);
+      return DDA_HIP_ERROR;
+    }
   } else {
-    rc = generate_dhat(op);
+    rc = generate_dhat(op, d.Bx, d.By, d.Bz, op->dhat);
     if (rc != DDA_OK) {
       free_op(op);
       return rc;
@@ -1180,6 +1395,45 @@ extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int 
   apply_cached_tuning(op);
   *out = op;
   return DDA_OK;
+}
+
+extern "C" dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out) {
+  return create_op(ctx, desc, precision, nullptr, nullptr, out);
+}
+
+extern "C" dda_status dda_op_create_cached(dda_ctx_t ctx, const dda_op_desc* desc, int precision,
+                                           dda_dhat_cache_t cache, dda_op_t* out) {
+  if (!cache || (desc && desc->dmatrix)) {
+    set_error("dda_op_create_cached: null cache, or a host tensor was given");
+    return DDA_INVALID_ARG;
+  }
+  if (ctx && ctx->device != cache->device) {
+    set_error("dda_op_create_cached: the cache belongs to another device");
+    return DDA_INVALID_ARG;
+  }
+  return create_op(ctx, desc, precision, cache, nullptr, out);
+}
+
+// The FP32 twin of an FP64 operator (same geometry; tensor converted from the
+// FP64 one, or from the same cache).
+dda_status dda::create_twin32(const dda_op_s* op, dda_op_t* out) {
+  dda_op_desc desc;
+  std::memset(&desc, 0, sizeof desc);
+  const MvDims& d = op->dims;
+  desc.box[0] = d.Bx;
+  desc.box[1] = d.By;
+  desc.box[2] = d.Bz;
+  desc.grid[0] = d.Gx;
+  desc.grid[1] = d.Gy;
+  desc.grid[2] = d.Gz;
+  desc.nv = d.nv;
+  desc.vec_stride = d.S;
+  desc.pos = op->pos.data();
+  desc.mat = op->mat.data();
+  desc.nmat = op->nmat;
+  desc.grid_unit = op->grid_unit;
+  desc.wave_num = op->wave_num;
+  return create_op(op->ctx, &desc, 32, op->dhat_cache, op, out);
 }
 
 extern "C" dda_status dda_op_destroy(dda_op_t op) {
@@ -1215,7 +1469,8 @@ static dda_status ensure_rhs_capacity(dda_op_s* op, int nrhs) {
   hipError_t e = hipMalloc(&a1, a1n * csz * nrhs);
   if (e == hipSuccess) e = hipMalloc(&a2, a2n * csz * nrhs);
   if (e == hipSuccess) e = hipMalloc(&ccs, 3ull * d.nmat * csz * nrhs);
-  if (e == hipSuccess) e = hipMemcpy(ccs, op->ccs, 3ull * d.nmat * csz * op->cap_rhs, hipMemcpyDeviceToDevice);
+  if (e == hipSuccess)
+    e = copy_sync(ccs, op->ccs, 3ull * d.nmat * csz * op->cap_rhs, hipMemcpyDeviceToDevice, op->ctx->stream);
   if (e != hipSuccess) {
     if (a1) (void)hipFree(a1);
     if (a2) (void)hipFree(a2);
@@ -1348,7 +1603,7 @@ extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
   const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
   std::vector<char> h(dn * csz);
   DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
-  DDA_TRY_HIP(hipMemcpy(h.data(), op->dhat, dn * csz, hipMemcpyDeviceToHost));
+  DDA_TRY_HIP(copy_sync(h.data(), op->dhat, dn * csz, hipMemcpyDeviceToHost, op->ctx->stream));
   const double inv_d3 = 1.0 / (op->grid_unit * op->grid_unit * op->grid_unit);
   for (int i = 0; i < d.Gx; i++)
     for (int k = 0; k < d.Gz; k++)

@@ -54,6 +54,7 @@ typedef struct dda_ctx_s* dda_ctx_t;
 typedef struct dda_op_s* dda_op_t;
 typedef struct dda_solver_s* dda_solver_t;
 typedef void* dda_stream_t; /* a hipStream_t; NULL selects the context's stream */
+typedef struct dda_dhat_cache_s* dda_dhat_cache_t;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* dda_last_error(void);
@@ -98,6 +99,26 @@ typedef struct dda_op_desc {
 /* GpuMatVec::create, DDA/GpuMatVec.hpp:85-86. precision is 32 or 64. */
 dda_status dda_op_create(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_op_t* out);
 dda_status dda_op_destroy(dda_op_t op);
+/* Device cache of FFT'd interaction tensors (no counterpart in the reference,
+ * which builds the tensor once per run on the CPU, DDA/DMatrixGpu.cpp:32-41).
+ * The d^3-scaled tensor depends only on the grid and kd = k d
+ * (DDA/DMatrixCpu.cpp:49-63), so operators of any geometry on the same grid
+ * and kd share one entry. Cached tensors clip the Green's function at G/2 per
+ * axis instead of at the box: the product is the same for every dipole inside
+ * the box (the embedding never pairs dipoles further apart than the box), to
+ * rounding. Entries are built on first use (FP32 entries from a cached FP64
+ * one when present) and evicted least-recently-used once unreferenced and
+ * over max_bytes. Thread-safe; one cache per device. */
+dda_status dda_dhat_cache_create(dda_ctx_t ctx, size_t max_bytes, dda_dhat_cache_t* out);
+/* Fails (INVALID_ARG) while operators still reference entries. */
+dda_status dda_dhat_cache_destroy(dda_dhat_cache_t cache);
+/* counts = {hits, misses, evictions}; bytes and entries held now. */
+dda_status dda_dhat_cache_stats(dda_dhat_cache_t cache, uint64_t counts[3], size_t* bytes, int* entries);
+/* dda_op_create with the tensor taken from (or added to) `cache`; desc->dmatrix
+ * must be NULL. The operator holds a reference until it is destroyed; the FP32
+ * twin of a mixed-mode solver over it uses the same cache. */
+dda_status dda_op_create_cached(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_dhat_cache_t cache,
+                                dda_op_t* out);
 /* Grid, vector stride and device bytes held by the operator. */
 dda_status dda_op_info(dda_op_t op, uint32_t grid[3], uint32_t* vec_stride, size_t* device_bytes);
 /* GpuMatVec::setCoupleConstants, DDA/GpuMatVec.cpp:139-142: sqrt of the
@@ -207,6 +228,57 @@ dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, 
  * each refinement step).
  * Writes min(n, iterations + 1) values and returns that count in *written. */
 dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n, uint64_t* written);
+
+/* ---------------------------------------------------------------------------
+ * Job server (no counterpart in the reference, which runs one geometry per
+ * process): scattering jobs are queued, admitted while their estimated device
+ * footprint fits the HBM budget, and run by `slots` workers, each on its own
+ * context and stream, so admitted jobs overlap on the GPU and a freed slot
+ * takes the next job at once (continuous batching). Queued jobs of the same
+ * particle (geometry, grid and wave number) are packed into one block solve of
+ * at most max_pack right-hand sides. Interaction tensors come from a device
+ * cache keyed by (grid, kd) (dda_dhat_cache_*). A job's result does not
+ * depend on what else runs: block solves give each right-hand side bitwise
+ * its single-solve result, and the cached tensor is the same for every job of
+ * that grid and kd.
+ * ------------------------------------------------------------------------- */
+typedef struct dda_server_s* dda_server_t;
+typedef struct dda_server_config {
+  int device;          /* GPU */
+  int slots;           /* concurrent block solves, 1..16, each on its own stream */
+  size_t hbm_budget;   /* bytes for admitted jobs plus the tensor cache (0 = 90% of HBM) */
+  size_t cache_bytes;  /* tensor cache limit, part of hbm_budget (0 = a quarter of it) */
+  int solver_kind;     /* dda_solver_kind */
+  int mode;            /* dda_solver_mode */
+  double eps;          /* relative residual */
+  uint64_t max_iter;   /* 0 = 3 * box volume */
+  int max_pack;        /* right-hand sides per block solve, >= 1 */
+} dda_server_config;
+typedef struct dda_job_desc {
+  dda_op_desc geometry;  /* as for dda_op_create; dmatrix must be NULL */
+  int nrhs;              /* incident fields of this job */
+  const double* cc_sqrt; /* [nrhs][nmat][3] complex double, one set per field */
+  const double* einc;    /* [nrhs][3][vec_stride] complex double */
+} dda_job_desc;
+typedef struct dda_job_result {
+  double t_submit, t_start, t_end; /* seconds since the server was created */
+  int32_t status;                  /* first non-OK dda_status of the job's fields */
+  int32_t packed_rhs;              /* right-hand sides of the block solve it ran in */
+  size_t footprint;                /* bytes admitted for that block solve */
+  int32_t cache_hit;               /* tensor found in the cache */
+  int32_t reserved;
+} dda_job_result;
+dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out);
+/* Copies the job's inputs and returns its id at once. */
+dda_status dda_server_submit(dda_server_t s, const dda_job_desc* job, uint64_t* id);
+/* Blocks until job `id` is done; fills pol_out ([nrhs][3][vec_stride]), info
+ * ([nrhs]) and res (all nullable) and forgets the job. Returns its status. */
+dda_status dda_server_wait(dda_server_t s, uint64_t id, double* pol_out, dda_solve_info* info, dda_job_result* res);
+/* counts = {submitted, completed, cache hits, cache misses, cache evictions};
+ * peak_bytes = the largest admitted footprint held at once. */
+dda_status dda_server_stats(dda_server_t s, uint64_t counts[5], size_t* peak_bytes);
+/* Runs the queued jobs to completion, then stops the workers. */
+dda_status dda_server_destroy(dda_server_t s);
 
 #ifdef __cplusplus
 }
