@@ -41,7 +41,11 @@ EXPORTS = [
     "dda_solver_set_inner_floor", "dda_solver_profile_read",
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
+    "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_incident_plane", "dda_cross_sections",
+    "dda_far_field", "dda_jones",
 ]
+POL_CM, POL_LDR, POL_LDR_AVGPOL = 0, 1, 2
+POL_KINDS = {"cm": POL_CM, "ldr": POL_LDR, "ldr-avgpol": POL_LDR_AVGPOL}
 
 
 class DDAError(RuntimeError):
@@ -145,6 +149,12 @@ def lib():
         L.dda_server_wait.argtypes = [vp, u64, dp, ctypes.POINTER(SolveInfo), ctypes.POINTER(JobResult)]
         L.dda_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(sz)]
         L.dda_server_destroy.argtypes = [vp]
+        L.dda_solver_solve_batch_dev.argtypes = [vp, vp, i32, dbl, vp, vp, ctypes.POINTER(SolveInfo)]
+        L.dda_couple_constants.argtypes = [i32, u32, dp, dbl, dbl, dp, dp, dp, dp]
+        L.dda_incident_plane.argtypes = [vp, i32, dp, dp, vp]
+        L.dda_cross_sections.argtypes = [vp, vp, vp, i32, dp, dp, dp]
+        L.dda_far_field.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int), dp, i32, dp]
+        L.dda_jones.argtypes = [vp, vp, i32, dp, dp, dp, u32, dbl, dbl, u32, dbl, dbl, dp]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("dda_last_error", "dda_fft_fit"):
@@ -439,6 +449,20 @@ class IterativeSolver:
 
     getPolVec = get_pol_vec
 
+    def solve_device(self, einc_dev, pol_dev, nrhs, eps, start_dev=None):
+        """Device-resident batched solve: einc_dev / pol_dev (/ start_dev) are
+        DeviceVectors (FP64) of nrhs * vec_size entries. Returns the infos."""
+        infos = (SolveInfo * nrhs)()
+        status = lib().dda_solver_solve_batch_dev(self.handle, einc_dev.ptr, int(nrhs), float(eps),
+                                                  start_dev.ptr if start_dev is not None else None, pol_dev.ptr,
+                                                  infos)
+        self.infos = [i.as_dict() for i in infos]
+        self.info = self.infos[0]
+        if status not in (OK, BREAKDOWN, MAX_ITER, STAGNATION, INVALID_ARG) or (
+                status == INVALID_ARG and not any(i["status"] == INVALID_ARG for i in self.infos)):
+            _check(status, "solve_device")
+        return self.infos
+
     def get_pol_vecs(self, einc, eps, start=None):
         """Solves a batch of incident fields [nrhs, 3*vec_stride] in lockstep.
 
@@ -601,3 +625,60 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def couple_constants(materials, grid_unit, kd, prop, pol, kind="ldr"):
+    """dda_couple_constants -> (cc, cc_sqrt), each [nmat, 3] complex."""
+    m = np.ascontiguousarray(np.asarray(materials, np.complex128).reshape(-1))
+    cc = np.zeros((m.size, 3), np.complex128)
+    cs = np.zeros((m.size, 3), np.complex128)
+    pr = np.ascontiguousarray(prop, np.float64)
+    pl = np.ascontiguousarray(pol, np.float64)
+    _check(lib().dda_couple_constants(POL_KINDS[kind], m.size, _dp(m.view(np.float64)), float(grid_unit), float(kd),
+                                      _dp(pr), _dp(pl), _dp(cc.view(np.float64)), _dp(cs.view(np.float64))),
+           "dda_couple_constants")
+    return cc, cs
+
+
+def incident_plane(op, props, pols, out=None):
+    """Plane-wave incident fields on the device for each (prop, pol) row -> DeviceVector [nrhs * vec_size]."""
+    pr = np.ascontiguousarray(np.asarray(props, np.float64).reshape(-1, 3))
+    pl = np.ascontiguousarray(np.asarray(pols, np.float64).reshape(-1, 3))
+    nrhs = pr.shape[0]
+    if out is None:
+        out = DeviceVector(op.ctx, nrhs * op.vec_size, 64)
+    _check(lib().dda_incident_plane(op.handle, nrhs, _dp(pr), _dp(pl), out.ptr), "dda_incident_plane")
+    return out
+
+
+def cross_sections(op, pol_dev, einc_dev, nrhs, cc=None):
+    """(cext [nrhs], cabs [nrhs] or None) from device vectors; cc [nrhs, nmat, 3] alpha."""
+    cext = np.zeros(nrhs)
+    cabs = np.zeros(nrhs) if cc is not None else None
+    c = None if cc is None else np.ascontiguousarray(np.asarray(cc, np.complex128).reshape(nrhs, -1, 3))
+    _check(lib().dda_cross_sections(op.handle, pol_dev.ptr, einc_dev.ptr, int(nrhs),
+                                    None if c is None else _dp(c.view(np.float64)), _dp(cext),
+                                    None if cabs is None else _dp(cabs)), "dda_cross_sections")
+    return cext, cabs
+
+
+def far_field(op, pol_dev, nrhs, dirs, rhs=None):
+    """Scattered far fields for directions dirs [nq, 3] (of right-hand side rhs[q]) -> [nq, 3] complex."""
+    d = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
+    nq = d.shape[0]
+    r = None if rhs is None else np.ascontiguousarray(rhs, np.int32)
+    out = np.zeros((nq, 3), np.complex128)
+    _check(lib().dda_far_field(op.handle, pol_dev.ptr, int(nrhs),
+                               None if r is None else r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(d), nq,
+                               _dp(out.view(np.float64))), "dda_far_field")
+    return out
+
+
+def jones(op, pol_dev, symmetric, prop, pol1, pol2, ntheta, theta0, dtheta, nphi, phi0, dphi):
+    """Jones matrices [ntheta * nphi, 2, 2] complex from the device solutions of both polarisations."""
+    out = np.zeros((ntheta * nphi, 2, 2), np.complex128)
+    a = [np.ascontiguousarray(v, np.float64) for v in (prop, pol1, pol2)]
+    _check(lib().dda_jones(op.handle, pol_dev.ptr, int(symmetric), _dp(a[0]), _dp(a[1]), _dp(a[2]), int(ntheta),
+                           float(theta0), float(dtheta), int(nphi), float(phi0), float(dphi),
+                           _dp(out.view(np.float64))), "dda_jones")
+    return out

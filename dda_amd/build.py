@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "_lib", "libdda_hip.so")
-SOURCES = ["api.hip", "matvec.hip", "krylov.hip", "server.cpp"]
+SOURCES = ["api.hip", "matvec.hip", "krylov.hip", "observe.hip", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

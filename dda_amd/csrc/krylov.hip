@@ -1452,8 +1452,9 @@ static dda_status zero_invalid(dda_solver_s* s, int k, const dda_solve_info* inf
   return DDA_OK;
 }
 
-extern "C" dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, double eps,
-                                             const double* start, double* pol_out, dda_solve_info* info) {
+// dev: einc, start and pol_out are device buffers (FP64), else host arrays.
+static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double eps, const void* start,
+                              void* pol_out, dda_solve_info* info, bool dev) {
   if (!s || !einc || !pol_out || !(eps > 0) || nrhs < 1) {
     set_error("dda_solver_solve: null argument, eps <= 0 or nrhs < 1");
     return DDA_INVALID_ARG;
@@ -1481,14 +1482,15 @@ extern "C" dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc,
     s->op32->prof = op->prof;
   }
   const size_t bytes = (size_t)nrhs * s->n * sizeof(double2);
-  DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, bytes, hipMemcpyHostToDevice, st));
-  if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + (size_t)s->cap * s->n, start, bytes, hipMemcpyHostToDevice, st));
+  const hipMemcpyKind in_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, bytes, in_kind, st));
+  if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + (size_t)s->cap * s->n, start, bytes, in_kind, st));
   std::vector<dda_solve_info> inf(nrhs);
   dda_status rc = s->mode == DDA_MODE_MIXED ? run_mixed(s, nrhs, eps, start != nullptr, inf.data())
                                             : run_fp64(s, nrhs, eps, start != nullptr, inf.data());
   if (rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED) return rc;
   DDA_TRY(zero_invalid(s, nrhs, inf.data()));
-  DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, bytes, hipMemcpyDeviceToHost, st));
+  DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, bytes, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
   DDA_TRY_HIP(hipStreamSynchronize(st));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int r = 0; r < nrhs; r++) {
@@ -1500,6 +1502,16 @@ extern "C" dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc,
   if (rc == DDA_STAGNATION) set_error("Too many iterations w/o increase");
   if (rc == DDA_INVALID_ARG) set_error("a right-hand side is zero");
   return rc;
+}
+
+extern "C" dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, double eps,
+                                             const double* start, double* pol_out, dda_solve_info* info) {
+  return solve_batch(s, einc, nrhs, eps, start, pol_out, info, false);
+}
+
+extern "C" dda_status dda_solver_solve_batch_dev(dda_solver_t s, const void* einc_dev, int nrhs, double eps,
+                                                 const void* start_dev, void* pol_dev, dda_solve_info* info) {
+  return solve_batch(s, einc_dev, nrhs, eps, start_dev, pol_dev, info, true);
 }
 
 extern "C" dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, const double* start,

@@ -1200,7 +1200,7 @@ static void free_op(dda_op_s* op) {
     op->dhat = nullptr;
   }
   void* bufs[] = {op->row_start, op->row_dip, op->row_meta, op->row_list, op->mat_dev, op->ccs, op->dhat, op->a1,
-                  op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y};
+                  op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y, op->pos_dev, op->obs_scratch};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete op;

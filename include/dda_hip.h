@@ -223,11 +223,59 @@ dda_status dda_solver_solve(dda_solver_t s, const double* einc, double eps, cons
  * side's result is bitwise the one it gets when solved alone. */
 dda_status dda_solver_solve_batch(dda_solver_t s, const double* einc, int nrhs, double eps, const double* start,
                                   double* pol_out, dda_solve_info* info);
+/* dda_solver_solve_batch with device-resident vectors: einc_dev, start_dev
+ * (nullable) and pol_dev are FP64 complex [nrhs][3][vec_stride] device
+ * buffers; nothing crosses PCIe. */
+dda_status dda_solver_solve_batch_dev(dda_solver_t s, const void* einc_dev, int nrhs, double eps, const void* start_dev,
+                                      void* pol_dev, dda_solve_info* info);
 /* residScale * |r|^2 after each iteration of the last solve (index 0 = start;
  * of right-hand side 0 after a batch; in mixed mode the FP64 |r|^2/|b|^2 after
  * each refinement step).
  * Writes min(n, iterations + 1) values and returns that count in *written. */
 dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n, uint64_t* written);
+
+/* ---------------------------------------------------------------------------
+ * Around the solve
+ * ------------------------------------------------------------------------- */
+typedef enum dda_polarizability {
+  DDA_POL_CM = 0,         /* Clausius-Mossotti, DDA/PolarizabilityDescription.cpp:65-77 */
+  DDA_POL_LDR = 1,        /* lattice dispersion relation, :85-110 */
+  DDA_POL_LDR_AVGPOL = 2  /* LDR with the polarisation-averaged S */
+} dda_polarizability;
+/* CoupleConstants (DDA/DDAParams.cpp:315-331): alpha and sqrt(alpha) per
+ * material and axis for refractive indices m ([nmat] complex), grid unit d,
+ * kd, propagation direction and incident polarisation; cc and cc_sqrt (the
+ * latter nullable) are [nmat][3] complex (m^3, m^(3/2)). Host-only. */
+dda_status dda_couple_constants(int kind, uint32_t nmat, const double* m, double grid_unit, double kd,
+                                const double prop[3], const double pol[3], double* cc, double* cc_sqrt);
+/* Plane-wave incident fields on the device, DDA/Beam.cpp:110-118:
+ * einc_dev[r][c][i] = exp(i k r_i . prop_r) pol_r[c] with r_i measured from the
+ * box centre (DipoleGeometry::moveToCenter); prop, pol host [nrhs][3];
+ * einc_dev FP64 complex [nrhs][3][vec_stride]. */
+dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* prop, const double* pol, void* einc_dev);
+/* Cext = 4 pi k sum Im(p . conj(E)) (DDA/Beam.cpp:58-68) and
+ * Cabs = 4 pi k sum (-Im(1/alpha) - 2/3 k^3) |p|^2 (DDA/AbsCross.cpp:27-48)
+ * per right-hand side from device vectors (FP64 [nrhs][3][vec_stride]); cc is
+ * host alpha [nrhs][nmat][3] complex (needed for cabs). cext / cabs: host
+ * [nrhs], either may be NULL. */
+dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const void* einc_dev, int nrhs, const double* cc,
+                              double* cext, double* cabs);
+/* Scattered far fields, DDA/CpuFieldCalculator.cpp:48-98: for query q, the
+ * field of right-hand side rhs[q] (NULL = all 0) in unit direction dirs[q]
+ * (host [nq][3]); out host [nq][3] complex. All queries run in one two-stage
+ * device reduction. */
+dda_status dda_far_field(dda_op_t op, const void* pol_dev, int nrhs, const int* rhs, const double* dirs, int nq,
+                         double* out);
+/* Jones matrices on theta0 + i dtheta (i < ntheta) x phi0 + j dphi (j < nphi)
+ * from the solutions of both polarisations (pol_dev [2][3][vec_stride]; with
+ * symmetric != 0 only polarisation 1, rotated as DDA/FarFieldCalc.cpp:167-168
+ * does), FarFieldCalc.cpp:119-174 and EMSim/JonesCalculus.cpp:74-86; out host
+ * [ntheta * nphi][2][2] complex, the matrix of getJonesMatrix
+ * (JonesCalculus.cpp:74-86; its text output lists s1..s4 = J[1][1], J[0][0],
+ * J[0][1], J[1][0], :88-116). */
+dda_status dda_jones(dda_op_t op, const void* pol_dev, int symmetric, const double prop[3], const double pol1[3],
+                     const double pol2[3], uint32_t ntheta, double theta0, double dtheta, uint32_t nphi, double phi0,
+                     double dphi, double* out);
 
 /* ---------------------------------------------------------------------------
  * Job server (no counterpart in the reference, which runs one geometry per
