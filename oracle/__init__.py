@@ -72,6 +72,7 @@ def lib():
                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_mie.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -83,6 +84,11 @@ def _p(a, t):
 def _cd(a):
     """complex128 array -> contiguous complex128 (viewed as interleaved doubles)."""
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def set_threads(n):
+    """OpenMP threads of the FFT matvec (1 = serial, as the reference's CPU path)."""
+    lib().orc_set_threads(int(n))
 
 
 def fft(data, inverse=False):

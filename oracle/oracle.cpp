@@ -43,6 +43,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef long double ld;
 
@@ -167,10 +170,21 @@ void matvec(const Prob& g, const std::complex<T>* arg, std::complex<T>* result, 
     uint8_t m = g.mat[i];
     for (int c = 0; c < 3; c++) xi(p[0], p[1], p[2], c) = cplx<T>(g.cc_sqrt, 3 * m + c) * mc(arg[i + c * S]);
   }
-  Fft<T> fx((int)Gx), fy((int)Gy), fz((int)Gz);
-  for (size_t l = 0; l < By * Bz * 3; l++) fx.run(&X[l * Gx], 1, false);
+  // Lines and x-planes are independent: with OpenMP each thread takes whole
+  // lines / planes with its own FFT work space, so the result does not depend
+  // on the thread count.
+#pragma omp parallel
+  {
+    Fft<T> fx((int)Gx);
+#pragma omp for schedule(static)
+    for (size_t l = 0; l < By * Bz * 3; l++) fx.run(&X[l * Gx], 1, false);
+  }
+#pragma omp parallel
+  {
+  Fft<T> fy((int)Gy), fz((int)Gz);
   // slices[gz][gy][3] Fortran (z fastest); slices_tr[gy][gz][3] (y fastest).
   std::vector<C> sl(Gz * Gy * 3), st(Gy * Gz * 3);
+#pragma omp for schedule(static)
   for (size_t i = 0; i < Gx; i++) {
     std::fill(sl.begin(), sl.end(), C(0));
     for (size_t j = 0; j < By; j++)
@@ -203,7 +217,13 @@ void matvec(const Prob& g, const std::complex<T>* arg, std::complex<T>* result, 
       for (size_t k = 0; k < Bz; k++)
         for (int c = 0; c < 3; c++) xi(i, j, k, c) = sl[k + Gz * (j + Gy * c)];
   }
-  for (size_t l = 0; l < By * Bz * 3; l++) fx.run(&X[l * Gx], 1, true);
+  }
+#pragma omp parallel
+  {
+    Fft<T> fx((int)Gx);
+#pragma omp for schedule(static)
+    for (size_t l = 0; l < By * Bz * 3; l++) fx.run(&X[l * Gx], 1, true);
+  }
   for (uint32_t i = g.nv; i < S; i++)
     for (int c = 0; c < 3; c++) result[i + c * S] = 0;
   for (uint32_t i = 0; i < g.nv; i++) {
@@ -734,6 +754,15 @@ void orc_green(double rx, double ry, double rz, double waveNum, int mu, int nu, 
   std::complex<ld> v = green<ld>(rx, ry, rz, waveNum, mu, nu);
   out[0] = (double)v.real();
   out[1] = (double)v.imag();
+}
+
+// Threads for the FFT matvec (OpenMP; 1 = the reference's serial CPU path).
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
 }
 
 // FFT matvec (MatVecCpu.cpp:67-150) in double (precision 64) or long double (80).

@@ -265,3 +265,18 @@ def test_geometry_counts():
     # sorted by (z, y, x)
     key = pos[:, 2].astype(np.int64) * 10 ** 8 + pos[:, 1] * 10 ** 4 + pos[:, 0]
     assert np.all(np.diff(key) > 0)
+
+
+def test_oracle_matvec_thread_count_independent():
+    """The OpenMP matvec (the multi-core CPU baseline) gives bitwise the serial result."""
+    import cases
+    from dda_amd import problem as P
+    g, params, cc, cs, prop, pol = cases.build("sphere_r7")
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params, seed=12)
+    O.set_threads(1)
+    y1 = orc.matvec(x, precision=64)
+    O.set_threads(4)
+    y4 = orc.matvec(x, precision=64)
+    O.set_threads(1)
+    assert np.array_equal(y1, y4)
