@@ -107,6 +107,31 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np):
     return 0
 
 
+def serving_bench(args, D):
+    """The serving stream: jobs of random shape (sphere / box / ellipsoid, largest
+    extent 16..96 dipoles), Re m in 1.2..2, Im m in 0..0.2, |m| kd = 0.5, random
+    orientation, both polarisations, mixed BiCGStab to eps. First all jobs at
+    once (capacity), then a fresh seeded stream with Poisson arrivals at 0.9x
+    that capacity; latency = completion - submission."""
+    from dda_amd import serving as S
+    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
+    try:
+        S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
+        burst = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=42))
+        rate = 0.9 * burst["jobs"] / burst["makespan_s"]
+        stream = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=43), rate=rate)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    strip = lambda r: {k: v for k, v in r.items()}
+    return {"workload": "%d jobs per run, sphere/box/ellipsoid, extent U{16..96}, Re m U(1.2,2), Im m U(0,0.2), "
+                        "|m|kd=0.5, random orientation, 2 polarisations, mixed BiCGStab to %g, 4 slots"
+                        % (args.serve_jobs, args.eps),
+            "capacity": strip(burst), "poisson_0.9x": dict(strip(stream), rate_jobs_per_s=rate),
+            "p50_ms": stream["p50_ms"], "p99_ms": stream["p99_ms"], "solves_per_s": stream["solves_per_s"],
+            "server": stats}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +144,8 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=24,
+                    help="jobs of the seeded serving stream (0 = skip the serving measurement)")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
                     help="instead of solves: one 512 MiB device copy (counter calibration), then K FP32 and "
                          "K FP64 matvecs; prints per-pass times (for rocprofv3 counter runs)")
@@ -296,6 +323,13 @@ def main():
         except Exception as e:
             cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": "failed: %s" % e}
 
+    serving = None
+    if args.serve_jobs > 0:
+        try:
+            serving = serving_bench(args, D)
+        except Exception as e:  # reported, not fatal: the headline is the C2 solve rate
+            serving = {"error": str(e)}
+
     info = ctx.info()
     line = {
         "metric": METRIC,
@@ -339,6 +373,7 @@ def main():
         "setup_s": t_setup,
         "device": info,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0

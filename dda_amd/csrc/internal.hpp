@@ -120,12 +120,14 @@ struct dda_op_s {
   struct Tune {
     int col_wide[2] = {-1, -1};  // forward / inverse Y pass
     int p3_wide = -1;            // Z pass with the tensor product
+    int x_wide[2] = {-1, -1};    // X passes (scatter + forward, inverse + gather)
   } tune;
   bool tuned = false;
   struct dda_dhat_cache_s* dhat_cache = nullptr;  // set when dhat is a cache entry (not owned)
   void* pos_dev = nullptr;      // [nv][3] positions, uploaded on first use by the observables
   void* obs_scratch = nullptr;  // observables' work space
   size_t obs_bytes = 0;  // dda_op_tune ran (an FP32 twin of a tuned operator is tuned too)
+  bool static_x = false;  // X passes with a compile-time plan (Gx has one)
   int cap_rhs = 1;   // right-hand sides the grid buffers hold
   int cc_rhs = 0;    // right-hand sides with couple constants set
   bool static_z = false;  // pass 3 uses a compile-time planned kernel

@@ -878,7 +878,9 @@ struct Core {
                                   reinterpret_cast<const void*>((intptr_t)nrhs),
                                   reinterpret_cast<const void*>((intptr_t)(op->tune.col_wide[0] * 4 +
                                                                            op->tune.col_wide[1] * 16 +
-                                                                           op->tune.p3_wide * 64 + 1024))};
+                                                                           op->tune.p3_wide * 64 +
+                                                                           op->tune.x_wide[0] * 256 +
+                                                                           op->tune.x_wide[1] * 1024 + 4096))};
     for (C* p : v) g.push_back(p);
     return g;
   }

@@ -410,6 +410,127 @@ __global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const ty
   }
 }
 
+// Pass 1 with a static plan of length N = Gx. A work-group takes LN consecutive
+// lines g = c * nrows + row of the A1 layout (contiguous in A1); thread (l, t),
+// t = tid mod TPC along the line, scatters line l's dipoles into the LDS tile,
+// then runs the transform from its first-stage points (only x < N/2 can hold
+// dipoles: the upper half is never read) and stores its outputs, TPC
+// consecutive points per line, coalesced.
+template <typename R, int N, bool W>
+__global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __restrict__ x,
+                                                typename CxT<R>::T* __restrict__ a1,
+                                                const uint32_t* __restrict__ row_start,
+                                                const uint32_t* __restrict__ row_dip,
+                                                const uint32_t* __restrict__ row_meta,
+                                                const typename CxT<R>::T* __restrict__ ccs, MvDims d,
+                                                const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl) {
+  typedef typename CxT<R>::T C;
+  typedef StaticPlan<N, W> P;
+  constexpr int PPT = P::PPT;
+  constexpr int TPC = N / PPT;
+  constexpr int LN = 256 / TPC;
+  constexpr int lp = (LN & 1) ? LN : LN + 1;
+  typedef typename P::Fwd F;
+  typedef typename Reverse<F>::type Out;
+  const int rhs = blockIdx.y;
+  if (stopped(fl, rhs)) return;
+  x += (long long)rhs * 3 * d.S;
+  a1 += (long long)rhs * 3 * d.nrows * d.Gx;
+  ccs += rhs * 3 * d.nmat;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int t = threadIdx.x % TPC, l = threadIdx.x / TPC;
+  const int nlines = 3 * d.nrows;
+  const int g = blockIdx.x * LN + l;
+  for (int i = threadIdx.x; i < (N / 2) * lp; i += blockDim.x) tile[i] = czero<C>();
+  __syncthreads();
+  if (g < nlines) {
+    const int c = g / d.nrows, row = g - c * d.nrows;
+    const uint32_t s0 = row_start[row], s1 = row_start[row + 1];
+    for (uint32_t q = s0 + t; q < s1; q += TPC) {
+      const uint32_t i = row_dip[q], meta = row_meta[q];
+      C v = x[(size_t)c * d.S + i];
+      if (conj) v = cconj(v);
+      tile[(meta & 0xffff) * lp + l] = cmul(ccs[3 * (meta >> 16) + c], v);
+    }
+  }
+  __syncthreads();
+  C v[PPT];
+#pragma unroll
+  for (int e = 0; e < PPT; e++) v[e] = upper_half<F>(e) ? czero<C>() : tile[first_point<N, PPT, F>(t, e) * lp + l];
+  __syncthreads();
+  RunStages<N, PPT, false, 1, C, F, true>::run(v, t, tile, lp, l, tw);
+  if (g < nlines) {
+    C* dst = a1 + (size_t)g * N;
+#pragma unroll
+    for (int e = 0; e < PPT; e++) dst[first_point<N, PPT, Out>(t, e)] = v[e];
+  }
+}
+
+// Pass 5 with a static plan: lines g = c * n_list + slot over the non-empty
+// rows; inverse transform from A1 (loads coalesced), the kept half (x < N/2)
+// through the tile, then line l's dipoles gathered by its TPC threads.
+template <typename R, int N, bool W>
+__global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __restrict__ a1,
+                                                const typename CxT<R>::T* __restrict__ x,
+                                                typename CxT<R>::T* __restrict__ y,
+                                                const uint32_t* __restrict__ row_list, int n_list,
+                                                const uint32_t* __restrict__ row_start,
+                                                const uint32_t* __restrict__ row_dip,
+                                                const uint32_t* __restrict__ row_meta,
+                                                const typename CxT<R>::T* __restrict__ ccs, MvDims d,
+                                                const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl) {
+  typedef typename CxT<R>::T C;
+  typedef StaticPlan<N, W> P;
+  constexpr int PPT = P::PPT;
+  constexpr int TPC = N / PPT;
+  constexpr int LN = 256 / TPC;
+  constexpr int lp = (LN & 1) ? LN : LN + 1;
+  typedef typename P::Fwd F;
+  typedef typename Reverse<F>::type Out;
+  const int rhs = blockIdx.y;
+  if (stopped(fl, rhs)) return;
+  a1 += (long long)rhs * 3 * d.nrows * d.Gx;
+  x += (long long)rhs * 3 * d.S;
+  y += (long long)rhs * 3 * d.S;
+  ccs += rhs * 3 * d.nmat;
+  extern __shared__ char smem[];
+  C* tile = reinterpret_cast<C*>(smem);
+  const int t = threadIdx.x % TPC, l = threadIdx.x / TPC;
+  const int g = blockIdx.x * LN + l;
+  const bool ok = g < 3 * n_list;
+  const int c = ok ? g / n_list : 0;
+  const int row = ok ? (int)row_list[g - c * n_list] : 0;
+  const C* src = a1 + ((size_t)c * d.nrows + row) * N;
+  C v[PPT];
+#pragma unroll
+  for (int e = 0; e < PPT; e++) v[e] = ok ? src[first_point<N, PPT, F>(t, e)] : czero<C>();
+  RunStages<N, PPT, true, 1, C, F>::run(v, t, tile, lp, l, tw);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < PPT; e++)
+    if (!upper_half<Out>(e)) tile[first_point<N, PPT, Out>(t, e) * lp + l] = v[e];
+  __syncthreads();
+  if (ok) {
+    const uint32_t s0 = row_start[row], s1 = row_start[row + 1];
+    for (uint32_t q = s0 + t; q < s1; q += TPC) {
+      const uint32_t i = row_dip[q], meta = row_meta[q];
+      C a = x[(size_t)c * d.S + i];
+      if (conj) a = cconj(a);
+      C r = cadd(cmul(ccs[3 * (meta >> 16) + c], tile[(meta & 0xffff) * lp + l]), a);
+      if (conj) r = cconj(r);
+      y[(size_t)c * d.S + i] = r;
+    }
+  }
+  if (blockIdx.x == 0) {
+    const int pad = d.S - d.nv;
+    for (int q = threadIdx.x; q < 3 * pad; q += blockDim.x) {
+      const int cc = q / pad;
+      y[(size_t)cc * d.S + d.nv + (q - cc * pad)] = czero<C>();
+    }
+  }
+}
+
 // Column FFT with a static plan (pass 2 / pass 4 and the tensor generation).
 // HIN: in_len <= N/2, the zero upper half of the input is skipped; HOUT:
 // out_len <= N/2, the upper half of the output is never formed.
@@ -725,6 +846,40 @@ dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, RhsFlags done, int 
                                         : launch_pass3_static_w<R, N, false>(op, st, done, nrhs);
 }
 
+template <typename R, int N, bool W>
+dda_status launch_x_static_w(dda_op_s* op, bool inverse, const void* x, void* y, int conj, hipStream_t st,
+                             RhsFlags done, int nrhs) {
+  typedef typename CxT<R>::T C;
+  constexpr int TPC = N / StaticPlan<N, W>::PPT;
+  constexpr int LN = 256 / TPC;
+  constexpr int lp = (LN & 1) ? LN : LN + 1;
+  const MvDims& d = op->dims;
+  const size_t lds = (size_t)N * lp * sizeof(C);
+  if (!inverse) {
+    auto k = k_pass1s<R, N, W>;
+    DDA_TRY(set_lds(k, lds));
+    hipLaunchKernelGGL(k, dim3((3 * d.nrows + LN - 1) / LN, nrhs), dim3(LN * TPC), lds, st, (const C*)x, (C*)op->a1,
+                       op->row_start, op->row_dip, op->row_meta, (const C*)op->ccs, d, (const C*)op->tw[0], conj,
+                       done);
+  } else {
+    auto k = k_pass5s<R, N, W>;
+    DDA_TRY(set_lds(k, lds));
+    const int nl = 3 * op->n_nonempty;
+    hipLaunchKernelGGL(k, dim3(nl > 0 ? (nl + LN - 1) / LN : 1, nrhs), dim3(LN * TPC), lds, st, (const C*)op->a1,
+                       (const C*)x, (C*)y, op->row_list, op->n_nonempty, op->row_start, op->row_dip, op->row_meta,
+                       (const C*)op->ccs, d, (const C*)op->tw[0], conj, done);
+  }
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
+template <typename R, int N>
+dda_status launch_x_static(dda_op_s* op, bool inverse, const void* x, void* y, int conj, hipStream_t st,
+                           RhsFlags done, int nrhs) {
+  return plan_wide<R>(op->tune.x_wide[inverse]) ? launch_x_static_w<R, N, true>(op, inverse, x, y, conj, st, done, nrhs)
+                                                : launch_x_static_w<R, N, false>(op, inverse, x, y, conj, st, done, nrhs);
+}
+
 // Lengths with a static plan; others use the run-time planned kernels.
 #define DDA_STATIC_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(512) X(1024)
 
@@ -821,7 +976,17 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   const int ppt_x = pts_x <= 8 * 256 ? 8 : 16;
   const size_t lds_x = (size_t)d.Gx * lds_stride(L) * sizeof(C);
   const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
-  {
+  if (op->static_x) {
+    PassTimer pt(op, st, 0);
+    switch (d.Gx) {
+#define X(N)                                                                \
+  case N:                                                                   \
+    DDA_TRY((launch_x_static<R, N>(op, false, x, nullptr, conj, st, done, nrhs))); \
+    break;
+      DDA_STATIC_SIZES(X)
+#undef X
+    }
+  } else {
     PassTimer pt(op, st, 0);
     dim3 grid((d.nrows + lr - 1) / lr, nrhs);
     if (ppt_x == 8) {
@@ -905,7 +1070,17 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
                                      op->tune.col_wide[1])));
   }
   // pass 5
-  {
+  if (op->static_x) {
+    PassTimer pt(op, st, 4);
+    switch (d.Gx) {
+#define X(N)                                                           \
+  case N:                                                              \
+    DDA_TRY((launch_x_static<R, N>(op, true, x, y, conj, st, done, nrhs))); \
+    break;
+      DDA_STATIC_SIZES(X)
+#undef X
+    }
+  } else {
     PassTimer pt(op, st, 4);
     dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1, nrhs);
     if (ppt_x == 8) {
@@ -1290,6 +1465,7 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
   }
   op->n_nonempty = (int)row_list.size();
   op->static_z = has_static_plan(d.Gz) && !std::getenv("DDA_GENERIC_FFT");
+  op->static_x = has_static_plan(d.Gx) && !std::getenv("DDA_GENERIC_FFT");
 
   // Launch shapes: X passes hold ~2048 points per work-group; column passes
   // take 128-byte row segments (8 complex doubles, 16 complex floats).
@@ -1678,13 +1854,32 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   const int reps = 5;
   char buf[160];
   dda_op_s::Tune best = op->tune;
-  for (int pass = 0; pass < 3; pass++) {
-    const bool is_static = pass == 1 ? op->static_z : has_static_plan(pass == 1 ? d.Gz : d.Gy);
+  void* xv = nullptr;  // dipole vectors for the X passes
+  if (op->static_x) {
+    DDA_TRY_HIP(hipMalloc(&xv, 2 * 3ull * d.S * csz * nrhs));
+    DDA_TRY_HIP(hipMemsetAsync(xv, 0, 2 * 3ull * d.S * csz * nrhs, st));
+  }
+  for (int pass = 0; pass < 5; pass++) {
+    const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
     double t[2] = {0, 0};
     for (int w = 0; w < 2; w++) {
       dda_status rc;
-      if (pass == 1) {
+      if (pass >= 3) {
+        op->tune.x_wide[pass - 3] = w;
+        char* xb = (char*)xv;
+        rc = time_pass(st, reps, [&]() {
+          switch (d.Gx) {
+#define X(N) \
+  case N:    \
+    return launch_x_static<R, N>(op, pass == 4, xb, xb + 3ull * d.S * csz * nrhs, 0, st, RhsFlags(), nrhs);
+            DDA_STATIC_SIZES(X)
+#undef X
+            default:
+              return DDA_NOT_SUPPORTED;
+          }
+        }, &t[w]);
+      } else if (pass == 1) {
         op->tune.p3_wide = w;
         rc = time_pass(st, reps, [&]() {
           switch (d.Gz) {
@@ -1717,19 +1912,26 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
       }
       if (rc != DDA_OK) {
         op->tune = best;
+        if (xv) (void)hipFree(xv);
         return rc;
       }
     }
     const int pick = t[1] <= t[0] ? 1 : 0;
     if (pass == 1)
       best.p3_wide = pick;
+    else if (pass >= 3)
+      best.x_wide[pass - 3] = pick;
     else
       best.col_wide[pass == 2] = pick;
+    static const char* names[5] = {"Y forward", "Z + tensor", "Y inverse", "X forward", "X inverse"};
     std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us -> %s", report->empty() ? "" : "; ",
-                  pass == 0 ? "Y forward" : pass == 1 ? "Z + tensor" : "Y inverse", t[0], t[1],
-                  pick ? "wide" : "narrow");
+                  names[pass], t[0], t[1], pick ? "wide" : "narrow");
     *report += buf;
     op->tune = best;
+  }
+  if (xv) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(xv);
   }
   if (report->empty()) *report = "no static-plan passes to tune at this grid";
   return DDA_OK;

@@ -62,10 +62,14 @@ def test_matvec_grid_choices(ctx, grid, pot):
     assert cases.rel_err(y, y_ref) <= FP64_TOL
 
 
-# Grid lengths with compile-time planned column kernels, each on the Y (passes
-# 2, 4) and the Z axis (pass 3); the 10x8x6 ellipsoid box embeds exactly in all.
+# Grid lengths with compile-time planned kernels, each on the Y (passes 2, 4),
+# the Z (pass 3) and the X axis (passes 1, 5); the 10x8x6 ellipsoid box embeds
+# exactly in all.
 STATIC_GRIDS = [(32, 64, 96), (32, 96, 64), (32, 128, 192), (32, 192, 128), (32, 256, 512), (32, 512, 256),
-                (32, 1024, 16), (32, 16, 1024)]
+                (32, 1024, 16), (32, 16, 1024),
+                # and on the X axis (passes 1, 5)
+                (64, 16, 16), (96, 16, 16), (128, 16, 16), (192, 16, 16), (256, 16, 16), (512, 16, 16),
+                (1024, 16, 16)]
 
 
 @pytest.mark.parametrize("grid", STATIC_GRIDS)
