@@ -107,6 +107,83 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np):
     return 0
 
 
+def large_bench(args, D, P, np):
+    """The largest single-GPU grid: sphere 512 dipoles across (~70M dipoles,
+    1024^3), m = 1.6+0.05i, |m| kd = 0.5. FP32 and FP64 matvec rates against
+    B_mv, the device memory held, and optionally mixed BiCGStab / QMR-CS to
+    1e-5 (one polarisation each)."""
+    m = 1.6 + 0.05j
+    d = 1e-7
+    kd = 0.5 / abs(m)
+    t0 = time.perf_counter()
+    g = P.sphere(256.0, m, d)
+    params = P.DDAParams(g, 2 * math.pi * d / kd)
+    t_geom = time.perf_counter() - t0
+    print("large: nv=%d grid=%s geometry %.1f s" % (params.nv, params.grid, t_geom), file=sys.stderr, flush=True)
+    ctx = D.Context(0)
+    prop, pol1, pol2 = P.beam_pols((0, 0, 1))
+    cc, cs = P.couple_constants(params, prop, pol1)
+    out = {"workload": "sphere 512 dipoles across (nv=%d), m=1.6+0.05i, |m|kd=0.5, grid %s"
+                       % (params.nv, "x".join(map(str, params.grid)))}
+    op64 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, d, params.wave_num, precision=64, grid=params.grid)
+    op64.set_couple_constants(cs)
+    for name, prec in (("fp32", 32), ("fp64", 64)):
+        t0 = time.perf_counter()
+        op = op64 if prec == 64 else D.GpuMatVec(ctx, g.pos, g.mat, g.box, d, params.wave_num, precision=32,
+                                                 grid=params.grid)
+        op.set_couple_constants(cs)
+        t_create = time.perf_counter() - t0
+        x = D.DeviceVector(ctx, params.vec_size, prec)
+        y = D.DeviceVector(ctx, params.vec_size, prec)
+        x.upload(P.random_dipole_vector(params))
+        op.apply(x, y)
+        ctx.synchronize()
+        op.profile(True)
+        for _ in range(5):
+            op.apply(x, y)
+        ms, cnt, _ = op.profile_read()
+        op.profile(False)
+        pav = [ms[i] / max(cnt[i], 1) for i in range(5)]
+        mv = float(sum(pav))
+        c = 8 if prec == 32 else 16
+        rows = np.unique(g.pos[:, 2].astype(np.int64) * g.box[1] + g.pos[:, 1]).size
+        pb = pass_bytes(params.nv, params.vec_stride, g.box, params.grid, rows, c)
+        bmv = b_mv(params.nv, g.box, params.grid, c)
+        out[name] = {"matvec_ms": mv, "pass_avg_us": [1e3 * v for v in pav],
+                     "pass_gbs": [pb[i] / (pav[i] * 1e-3) / 1e9 for i in range(5)],
+                     "gbs_vs_bmv": bmv / (mv * 1e-3) / 1e9, "frac_vs_bmv": bmv / (mv * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "gbs_algorithmic": sum(pb) / (mv * 1e-3) / 1e9, "device_bytes": op.device_bytes,
+                     "create_s": t_create}
+        print("large %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
+        x.close()
+        y.close()
+        if prec == 32:
+            op.close()
+    if args.large_solves:
+        einc = P.einc_plane(params, prop, pol1)
+        out["solves"] = {}
+        for kind in ("bicgstab", "qmr"):
+            s = D.SOLVER_CLASSES[kind](op64, mixed=True, max_iter=20000)
+            t0 = time.perf_counter()
+            try:
+                p = s.get_pol_vec(einc, 1e-5)
+                status = "ok"
+            except D.DDAError as e:
+                p, status = None, str(e)
+            el = time.perf_counter() - t0
+            info = dict(s.info)
+            res = {"seconds": el, "status": status, "iterations": info["iterations"],
+                   "refinements": info["refinements"], "true_rel_residual": info["true_rel_residual"]}
+            if p is not None:
+                res["cext"] = P.ext_cross(params, p, einc)
+            out["solves"][kind] = res
+            print("large solve %s: %s" % (kind, json.dumps(res)), file=sys.stderr, flush=True)
+            s.close()
+    out["device_bytes_total"] = op64.device_bytes + out["fp32"]["device_bytes"]
+    print(json.dumps(out))
+    return 0
+
+
 def serving_bench(args, D):
     """The serving stream: jobs of random shape (sphere / box / ellipsoid, largest
     extent 16..96 dipoles), Re m in 1.2..2, Im m in 0..0.2, |m| kd = 0.5, random
@@ -144,6 +221,9 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="instead of the headline: the 1024^3 sphere (512 across) matvec rates and memory")
+    ap.add_argument("--large-solves", action="store_true", help="with --large: also mixed solves to 1e-5")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the seeded serving stream (0 = skip the serving measurement)")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
@@ -158,6 +238,8 @@ def main():
     import numpy as np
     import dda_amd as D
     from dda_amd import problem as P
+    if args.large:
+        return large_bench(args, D, P, np)
 
     m = 1.5 + 0.1j
     d = 1e-7

@@ -55,6 +55,7 @@ extern "C" dda_status dda_ctx_synchronize(dda_ctx_t ctx) {
     set_error("dda_ctx_synchronize: null context");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
   return DDA_OK;
 }
@@ -88,6 +89,7 @@ extern "C" dda_status dda_free(dda_ctx_t ctx, void* ptr) {
     set_error("dda_free: null context");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   if (ptr) {
     DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
     DDA_TRY_HIP(hipFree(ptr));
@@ -100,6 +102,7 @@ extern "C" dda_status dda_memcpy_h2d(dda_ctx_t ctx, void* dst, const void* src, 
     set_error("dda_memcpy_h2d: null argument");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   DDA_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
   return DDA_OK;
@@ -110,6 +113,7 @@ extern "C" dda_status dda_memcpy_d2h(dda_ctx_t ctx, void* dst, const void* src, 
     set_error("dda_memcpy_d2h: null argument");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   DDA_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
   return DDA_OK;
@@ -120,6 +124,7 @@ extern "C" dda_status dda_memcpy_d2d(dda_ctx_t ctx, void* dst, const void* src, 
     set_error("dda_memcpy_d2d: null argument");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   DDA_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   DDA_TRY_HIP(hipStreamSynchronize(ctx->stream));
   return DDA_OK;

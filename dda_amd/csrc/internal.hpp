@@ -78,6 +78,22 @@ struct dda_ctx_s {
   hipDeviceProp_t prop;
 };
 
+namespace dda {
+// Makes the context's device current on the calling thread (handles may be
+// used from any thread, one at a time).
+inline dda_status bind_device(const dda_ctx_s* c) {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != c->device) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) {
+      set_error(std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+      return DDA_HIP_ERROR;
+    }
+  }
+  return DDA_OK;
+}
+}  // namespace dda
+
 struct dda_op_s {
   dda_ctx_s* ctx = nullptr;
   int prec = 64;

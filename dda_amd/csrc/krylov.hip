@@ -1406,6 +1406,7 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
     set_error("dda_solver_create: null argument, unknown solver kind or mode");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   if (op->prec != 64) {
     set_error("dda_solver_create: the operator must be FP64 (mixed mode builds its own FP32 twin)");
     return DDA_INVALID_ARG;
@@ -1461,6 +1462,7 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
     set_error("dda_solver_solve: null argument, eps <= 0 or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(s->op->ctx));
   dda_op_s* op = s->op;
   if (!op->cc_set) {
     set_error("dda_solver_solve: couple constants have not been set");

@@ -1391,6 +1391,7 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
     set_error("dda_op_create: null argument or precision not 32/64");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(ctx));
   *out = nullptr;
   const dda_op_desc& ds = *desc;
   if (ds.nv == 0 || !ds.pos || ds.nmat == 0 || ds.nmat > 256 || !(ds.grid_unit > 0) || !(ds.wave_num > 0)) {
@@ -1692,6 +1693,7 @@ extern "C" dda_status dda_op_set_cc_sqrt(dda_op_t op, const double* cc_sqrt) {
     set_error("dda_op_set_cc_sqrt: null argument");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 6ull * op->nmat);
   DDA_TRY(upload_ccs(op, cc_sqrt, 0, 1));
   op->cc_set = true;
@@ -1704,6 +1706,7 @@ extern "C" dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqr
     set_error("dda_op_set_cc_sqrt_batch: null argument or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   DDA_TRY(ensure_rhs_capacity(op, nrhs));
   DDA_TRY(upload_ccs(op, cc_sqrt, 0, nrhs));
   op->cc_sqrt_host.assign(cc_sqrt, cc_sqrt + 6ull * op->nmat * nrhs);
@@ -1722,6 +1725,7 @@ extern "C" dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y
     set_error("dda_op_apply: null or aliasing vectors, or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   if (!op->cc_set || nrhs > op->cc_rhs) {
     set_error("dda_op_apply: couple constants not set for every right-hand side");
     return DDA_INVALID_ARG;
@@ -1736,6 +1740,7 @@ extern "C" dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y,
     set_error("dda_op_apply_host: null argument");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   const size_t n = 3ull * op->dims.S;
   const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
   if (!op->stage_x) {
@@ -1951,6 +1956,7 @@ extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t re
     set_error("dda_op_tune: null operator or nrhs outside 1 .. the operator's capacity");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   std::string rep;
   DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
   dda_status rc = op->prec == 64 ? tune_impl<double>(op, nrhs, &rep) : tune_impl<float>(op, nrhs, &rep);

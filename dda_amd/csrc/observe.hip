@@ -235,6 +235,7 @@ extern "C" dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* pr
     set_error("dda_incident_plane: null argument or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   DDA_TRY(positions_on_device(op));
   void* scratch = nullptr;
   DDA_TRY(obs_scratch(op, 6ull * nrhs * sizeof(double), &scratch));
@@ -260,6 +261,7 @@ extern "C" dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const
     set_error("dda_cross_sections: null argument or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   const MvDims& d = op->dims;
   const int nb = std::max(1, std::min(256, (d.nv + 4095) / 4096));
   const double k = op->wave_num;
@@ -301,6 +303,7 @@ extern "C" dda_status dda_far_field(dda_op_t op, const void* pol_dev, int nrhs, 
     set_error("dda_far_field: null argument, nrhs < 1 or no directions");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(bind_device(op->ctx));
   for (int i = 0; i < nq; i++)
     if (rhs && (rhs[i] < 0 || rhs[i] >= nrhs)) {
       set_error("dda_far_field: right-hand side index out of range");

@@ -153,8 +153,9 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
 }
 
 void dda_server_s::worker(int slot) {
-  dda_ctx_t ctx = slot == 0 ? ctx0 : nullptr;
-  if (!ctx && dda_ctx_create(cfg.device, &ctx) != DDA_OK) ctx = nullptr;
+  (void)slot;
+  dda_ctx_t ctx = nullptr;  // each slot: its own stream, its device bound on this thread
+  if (dda_ctx_create(cfg.device, &ctx) != DDA_OK) ctx = nullptr;
   for (;;) {
     std::vector<Job*> pack;
     size_t foot = 0;
@@ -212,7 +213,7 @@ void dda_server_s::worker(int slot) {
     cv_done.notify_all();
     cv_work.notify_all();
   }
-  if (ctx && ctx != ctx0) dda_ctx_destroy(ctx);
+  if (ctx) dda_ctx_destroy(ctx);
 }
 
 extern "C" dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out) {
