@@ -43,10 +43,10 @@ def _rotation(q):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def make_job(rng, shape=None, dim=None, m=None):
+def make_job(rng, shape=None, dim=None, m=None, other=None):
     shape = shape or ["sphere", "box", "ellipsoid"][rng.integers(3)]
     dim = int(dim or rng.integers(16, 97))
-    other = [max(1, int(round(dim * rng.uniform(0.5, 1.0)))) for _ in range(2)]
+    other = list(other) if other is not None else [max(1, int(round(dim * rng.uniform(0.5, 1.0)))) for _ in range(2)]
     m = m if m is not None else complex(rng.uniform(1.2, 2.0), rng.uniform(0.0, 0.2))
     if shape == "sphere":
         g = P.sphere(dim / 2.0, m, GRID_UNIT)

@@ -32,9 +32,12 @@ def test_incident_and_device_solve(ctx):
     g, params, op, s, einc, ccs, props, pols, e_dev, p_dev = _solved(ctx, dirs=((0, 0, 1), (0.6, 0, 0.8)))
     e = e_dev.download().reshape(len(ccs), -1)
     np.testing.assert_allclose(e, einc, rtol=0, atol=1e-14)
-    # the device-resident solve is bitwise the host-array one
-    p_host, _ = s.get_pol_vecs(einc, 1e-10)
+    # the device-resident solve is bitwise the host-array one on the same input
+    p_host, _ = s.get_pol_vecs(e, 1e-10)
     assert np.array_equal(p_dev.download().reshape(len(ccs), -1), p_host)
+    # and agrees with the solve of the host-built field to the solver tolerance
+    p_host2, _ = s.get_pol_vecs(einc, 1e-10)
+    assert cases.rel_err(p_host, p_host2) <= 1e-8
 
 
 def test_cross_sections_match_oracle(ctx):

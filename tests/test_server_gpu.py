@@ -83,8 +83,10 @@ def test_server_admission_and_cache():
     """A budget that holds one job at a time serialises the jobs; all finish;
     jobs on the same grid and kd share the cached tensor."""
     rng = np.random.default_rng(11)
-    jobs = [S.make_job(rng, "box", 10, 1.4 + 0.02j) for _ in range(2)]
-    jobs += [S.make_job(rng, "ellipsoid", 10, 1.4 + 0.02j) for _ in range(2)]
+    # box 10x8x6 and the ellipsoid in it: different particles, one grid and kd
+    jobs = [S.make_job(rng, "box", 10, 1.4 + 0.02j, other=(8, 6)) for _ in range(2)]
+    jobs += [S.make_job(rng, "ellipsoid", 10, 1.4 + 0.02j, other=(8, 6)) for _ in range(2)]
+    assert jobs[0].params.grid == jobs[2].params.grid
     one = D.Server(slots=4, max_pack=2, eps=1e-6)
     foot = one.wait(S.submit(one, jobs[0]))[2]["footprint"]
     one.close()
