@@ -62,12 +62,15 @@ def measured_traffic(diameter, grid, prec, pass_index, nrhs=1):
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size),
     or None when no measurement exists for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic_matvec_sphere128.md")
-    if nrhs != 1 or diameter != 128 or tuple(grid) != (256, 256, 256) or not os.path.exists(path):
+    if diameter != 128 or tuple(grid) != (256, 256, 256) or not os.path.exists(path):
         return None
     text = open(path).read()
     blob = text.split("```json", 1)[1].split("```", 1)[0]
     data = json.loads(blob)
-    return data[prec][pass_index]["traffic_bytes"]
+    key = prec if nrhs == 1 else "%s_rhs%d" % (prec, nrhs)
+    if key not in data:
+        return None
+    return data[key][pass_index]["traffic_bytes"]
 
 
 def cpu_info():
@@ -82,24 +85,25 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np):
-    """Copy 512 MiB device to device once, then k matvecs in FP32 and in FP64."""
+def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np, nrhs=1):
+    """Copy 512 MiB device to device once, then k matvecs in FP32 and in FP64
+    (each over nrhs right-hand sides as one block launch per pass)."""
     n = (512 << 20) // 16
     a, b = D.DeviceVector(ctx, n, 64), D.DeviceVector(ctx, n, 64)
     b.copy_from(a)
-    out = {"copy_bytes": 512 << 20}
+    out = {"copy_bytes": 512 << 20, "nrhs": nrhs}
     op32 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=32, grid=params.grid)
     for name, op in (("fp32", op32), ("fp64", op64)):
-        op.set_couple_constants(rhs[0][2])
+        op.set_couple_constants_batch(np.array([rhs[r % len(rhs)][2] for r in range(nrhs)]))
         prec = 32 if name == "fp32" else 64
-        x = D.DeviceVector(ctx, params.vec_size, prec)
-        y = D.DeviceVector(ctx, params.vec_size, prec)
-        x.upload(P.random_dipole_vector(params))
-        op.apply(x, y)
+        x = D.DeviceVector(ctx, nrhs * params.vec_size, prec)
+        y = D.DeviceVector(ctx, nrhs * params.vec_size, prec)
+        x.upload(np.concatenate([P.random_dipole_vector(params, seed=r + 1) for r in range(nrhs)]))
+        op.apply_batch(x, y, nrhs)
         ctx.synchronize()
         op.profile(True)
         for _ in range(k):
-            op.apply(x, y)
+            op.apply_batch(x, y, nrhs)
         ms, cnt, _ = op.profile_read()
         op.profile(False)
         out[name + "_pass_avg_us"] = [1e3 * ms[i] / max(cnt[i], 1) for i in range(5)]
@@ -221,6 +225,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-rhs", type=int, default=1, help="with --matvec-profile: right-hand sides per launch")
     ap.add_argument("--large", action="store_true",
                     help="instead of the headline: the 1024^3 sphere (512 across) matvec rates and memory")
     ap.add_argument("--large-solves", action="store_true", help="with --large: also mixed solves to 1e-5")
@@ -261,7 +266,7 @@ def main():
     t_setup = time.perf_counter() - t0
     modes = [m.strip() for m in args.runs.split(",") if m.strip()]
     if args.matvec_profile:
-        return matvec_profile(ctx, op, g, params, rhs, args.matvec_profile, D, P, np)
+        return matvec_profile(ctx, op, g, params, rhs, args.matvec_profile, D, P, np, args.profile_rhs)
 
     def run_mode(run):
         parts = run.split(":")
