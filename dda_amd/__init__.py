@@ -89,10 +89,10 @@ class JobDesc(ctypes.Structure):
 class JobResult(ctypes.Structure):
     _fields_ = [("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("status", ctypes.c_int32), ("packed_rhs", ctypes.c_int32), ("footprint", ctypes.c_size_t),
-                ("cache_hit", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("cache_hit", ctypes.c_int32), ("iterations", ctypes.c_int32), ("setup_s", ctypes.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None

@@ -127,10 +127,12 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
   d.wave_num = j0.wave_num;
   dda_op_t op = nullptr;
   dda_solver_t s = nullptr;
+  const double t0 = now();
   dda_status rc = dda_op_create_cached(ctx, &d, 64, cache, &op);
   const bool hit = rc == DDA_OK && dda::last_tensor_cache_hit();
   if (rc == DDA_OK) rc = dda_op_set_cc_sqrt_batch(op, cc.data(), k);
   if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
+  const double setup = now() - t0;
   if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
   const bool call_failed = rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED ||
                            (rc == DDA_INVALID_ARG && !s);
@@ -147,6 +149,9 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
     j->res.status = call_failed ? (int)rc : st;
     j->res.packed_rhs = k;
     j->res.cache_hit = hit ? 1 : 0;
+    j->res.setup_s = setup;
+    j->res.iterations = 0;
+    for (int r = 0; r < j->nrhs; r++) j->res.iterations = std::max<int>(j->res.iterations, (int)info[r0 + r].iterations);
     j->error = err;
     r0 += j->nrhs;
   }

@@ -314,7 +314,8 @@ typedef struct dda_job_result {
   int32_t packed_rhs;              /* right-hand sides of the block solve it ran in */
   size_t footprint;                /* bytes admitted for that block solve */
   int32_t cache_hit;               /* tensor found in the cache */
-  int32_t reserved;
+  int32_t iterations;              /* most Krylov iterations among the job's fields */
+  double setup_s;                  /* operator, tensor and solver set-up of the block solve */
 } dda_job_result;
 dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out);
 /* Copies the job's inputs and returns its id at once. */
