@@ -33,6 +33,7 @@ SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_
 EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
+    "dda_fft_fit_static",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_set_cc_sqrt_batch",
     "dda_op_apply", "dda_op_apply_batch", "dda_op_tune",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
@@ -119,6 +120,8 @@ def lib():
         L.dda_memcpy_d2h.argtypes = [vp, vp, vp, sz]
         L.dda_memcpy_d2d.argtypes = [vp, vp, vp, sz]
         L.dda_fft_fit.argtypes = [u32, u32, i32]
+        L.dda_fft_fit_static.argtypes = [u32]
+        L.dda_fft_fit_static.restype = u32
         L.dda_fft_fit.restype = u32
         L.dda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), i32, pp]
         L.dda_op_destroy.argtypes = [vp]

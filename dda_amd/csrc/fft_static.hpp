@@ -225,6 +225,10 @@ template <int N, bool WIDE> struct StaticPlan { static constexpr bool exists = f
     static constexpr int PPT = P;                         \
     typedef RadixList<__VA_ARGS__> Fwd;                   \
   };
+DDA_PLAN(32, false, 8, 8, 4)
+DDA_PLAN(32, true, 16, 16, 2)
+DDA_PLAN(48, false, 12, 12, 4)
+DDA_PLAN(48, true, 12, 12, 4)
 DDA_PLAN(64, false, 8, 8, 8)
 DDA_PLAN(64, true, 8, 8, 8)
 DDA_PLAN(96, false, 12, 6, 4, 4)

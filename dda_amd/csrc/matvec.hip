@@ -784,7 +784,9 @@ template <typename R, int N, bool W>
 constexpr int static_cols() {
   constexpr int pref = sizeof(R) == 8 ? 8 : 16;
   constexpr int tpc = N / StaticPlan<N, W>::PPT;
-  return pref * tpc <= 256 ? pref : 256 / tpc;
+  // at least one full wavefront per work-group, at most 256 threads
+  constexpr int t = pref * tpc < 64 ? 64 / tpc : pref;
+  return t * tpc <= 256 ? t : 256 / tpc;
 }
 
 // Plan width of a static kernel: 1 = wide (16 points per thread where the plan
@@ -881,7 +883,7 @@ dda_status launch_x_static(dda_op_s* op, bool inverse, const void* x, void* y, i
 }
 
 // Lengths with a static plan; others use the run-time planned kernels.
-#define DDA_STATIC_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(512) X(1024)
+#define DDA_STATIC_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512) X(1024)
 
 inline bool has_static_plan(int n) {
   switch (n) {
@@ -1340,6 +1342,17 @@ extern "C" dda_status dda_dhat_cache_stats(dda_dhat_cache_t c, uint64_t counts[3
   if (bytes) *bytes = c->bytes;
   if (entries) *entries = (int)c->map.size();
   return DDA_OK;
+}
+
+extern "C" uint32_t dda_fft_fit_static(uint32_t x) {
+  static const uint32_t sizes[] = {
+#define X(N) N,
+      DDA_STATIC_SIZES(X)
+#undef X
+  };
+  for (uint32_t n : sizes)
+    if (n >= x) return n;
+  return dda_fft_fit(x, 0, 1);
 }
 
 extern "C" uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot) {

@@ -266,7 +266,7 @@ extern "C" dda_status dda_server_submit(dda_server_t s, const dda_job_desc* job,
   std::unique_ptr<Job> j(new Job());
   for (int a = 0; a < 3; a++) {
     j->box[a] = g.box[a];
-    j->grid[a] = g.grid[a] ? g.grid[a] : dda_fft_fit(2 * g.box[a], 0, 1);
+    j->grid[a] = g.grid[a] ? g.grid[a] : dda_fft_fit_static(2 * g.box[a]);
     if (g.box[a] == 0 || j->grid[a] < 2 * g.box[a] || j->grid[a] % 2) {
       dda::set_error("dda_server_submit: grid must be even and at least twice the box");
       return DDA_INVALID_ARG;

@@ -81,10 +81,12 @@ def make_stream(n, seed, rate=None):
     return jobs
 
 
-def submit(server, job):
+def submit(server, job, grid=(0, 0, 0)):
+    """Submits a job; grid 0 lets the server pick per axis the smallest length
+    with compile-time planned kernels (dda_fft_fit_static)."""
     g = job.geometry
     return server.submit(g.pos, g.mat, g.box, g.grid_unit, job.params.wave_num, job.einc, job.cc_sqrt,
-                         nmat=len(g.materials), grid=job.params.grid)
+                         nmat=len(g.materials), grid=grid)
 
 
 def run_stream(server, jobs, rate=None):

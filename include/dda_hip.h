@@ -76,6 +76,12 @@ dda_status dda_memcpy_d2d(dda_ctx_t ctx, void* dst, const void* src, size_t byte
 /* FFT grid size for a box extent of x dipoles: fftFit, DDA/DDAParams.cpp:45-90.
  * x is 2*box; override != 0 returns override (0 if it is smaller than x). */
 uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot);
+/* The smallest length >= x that has compile-time planned kernels (32, 48, 64,
+ * 96, 128, 192, 256, 512, 1024), else dda_fft_fit's smooth rule: the faster
+ * grid for a box extent of x/2 (any grid of at least twice the box gives the
+ * same product, to rounding). The job server uses it for jobs without a grid;
+ * the reference's GPU path likewise rounds to powers of two. */
+uint32_t dda_fft_fit_static(uint32_t x);
 
 /* Operator description: the parts of DDAParams/DipoleGeometry the matvec reads
  * (DDA/GpuMatVec.cpp:88-136). */
@@ -284,7 +290,8 @@ dda_status dda_jones(dda_op_t op, const void* pol_dev, int symmetric, const doub
  * context and stream, so admitted jobs overlap on the GPU and a freed slot
  * takes the next job at once (continuous batching). Queued jobs of the same
  * particle (geometry, grid and wave number) are packed into one block solve of
- * at most max_pack right-hand sides. Interaction tensors come from a device
+ * at most max_pack right-hand sides. Jobs without a grid get
+ * dda_fft_fit_static(2 box) per axis. Interaction tensors come from a device
  * cache keyed by (grid, kd) (dda_dhat_cache_*). A job's result does not
  * depend on what else runs: block solves give each right-hand side bitwise
  * its single-solve result, and the cached tensor is the same for every job of

@@ -42,3 +42,11 @@ def test_dippol_and_jones_text_round_trip(tmp_path):
     assert np.allclose(t2, np.degrees(th)) and np.allclose(p2, np.degrees(ph))
     assert O.field_diff(J2, J) < 1e-9
     assert O.field_diff(J, J) == 0.0 and abs(O.field_diff(J, 2 * J) - math.sqrt(0.4)) < 1e-12
+
+
+def test_fft_fit_static():
+    """The fast-grid rule: smallest compile-time planned length >= x, else the smooth fftFit."""
+    want = {20: 32, 32: 32, 33: 48, 50: 64, 97: 128, 100: 128, 130: 192, 200: 256, 300: 512, 600: 1024,
+            1100: 1120}
+    for x, n in want.items():
+        assert D.lib().dda_fft_fit_static(x) == n, x

@@ -69,7 +69,7 @@ STATIC_GRIDS = [(32, 64, 96), (32, 96, 64), (32, 128, 192), (32, 192, 128), (32,
                 (32, 1024, 16), (32, 16, 1024),
                 # and on the X axis (passes 1, 5)
                 (64, 16, 16), (96, 16, 16), (128, 16, 16), (192, 16, 16), (256, 16, 16), (512, 16, 16),
-                (1024, 16, 16)]
+                (1024, 16, 16), (32, 32, 32), (48, 48, 48)]
 
 
 @pytest.mark.parametrize("grid", STATIC_GRIDS)
