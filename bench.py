@@ -166,8 +166,8 @@ def large_bench(args, D, P, np):
     if args.large_solves:
         einc = P.einc_plane(params, prop, pol1)
         out["solves"] = {}
-        for kind in ("bicgstab", "qmr"):
-            s = D.SOLVER_CLASSES[kind](op64, mixed=True, max_iter=20000)
+        for kind in args.large_solvers.split(","):
+            s = D.SOLVER_CLASSES[kind](op64, mixed=True, max_iter=args.large_max_iter)
             t0 = time.perf_counter()
             try:
                 p = s.get_pol_vec(einc, 1e-5)
@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="instead of the headline: the 1024^3 sphere (512 across) matvec rates and memory")
     ap.add_argument("--large-solves", action="store_true", help="with --large: also mixed solves to 1e-5")
+    ap.add_argument("--large-solvers", default="bicgstab,qmr")
+    ap.add_argument("--large-max-iter", type=int, default=20000)
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the seeded serving stream (0 = skip the serving measurement)")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",

@@ -330,7 +330,8 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
 
 // Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
 template <typename R, int N, int T, bool W>
-__global__ __launch_bounds__(256) void k_pass3s(typename CxT<R>::T* a2, const typename CxT<R>::T* __restrict__ dhat,
+__global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_pass3s(typename CxT<R>::T* a2,
+                                                                              const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
                                                 int nper, int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
@@ -535,7 +536,8 @@ __global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __rest
 // HIN: in_len <= N/2, the zero upper half of the input is skipped; HOUT:
 // out_len <= N/2, the upper half of the output is never formed.
 template <typename R, int N, int T, bool W, bool INV, bool HIN, bool HOUT>
-__global__ __launch_bounds__(256) void k_colfft_s(const typename CxT<R>::T* in, typename CxT<R>::T* out, ColParams cp,
+__global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_colfft_s(const typename CxT<R>::T* in,
+                                                                                typename CxT<R>::T* out, ColParams cp,
                                                   const typename CxT<R>::T* __restrict__ tw, RhsFlags fl) {
   typedef typename CxT<R>::T C;
   typedef StaticPlan<N, W> P;
@@ -780,13 +782,17 @@ dda_status launch_row(void* data, long long nlines, const FftPlan& p, const void
 
 // Columns per work-group for a static plan: 128-byte row segments, at most 256
 // threads.
-template <typename R, int N, bool W>
+// Long columns (512, 1024 points) keep 128-byte segments with larger
+// work-groups: up to 1024 threads for a single-component transform, 512 for
+// the Z pass, whose threads hold three components.
+template <typename R, int N, bool W, int MAXT = 256>
 constexpr int static_cols() {
   constexpr int pref = sizeof(R) == 8 ? 8 : 16;
   constexpr int tpc = N / StaticPlan<N, W>::PPT;
-  // at least one full wavefront per work-group, at most 256 threads
+  constexpr int cap = N >= 512 ? MAXT : 256;
+  // at least one full wavefront per work-group
   constexpr int t = pref * tpc < 64 ? 64 / tpc : pref;
-  return t * tpc <= 256 ? t : 256 / tpc;
+  return t * tpc <= cap ? t : cap / tpc;
 }
 
 // Plan width of a static kernel: 1 = wide (16 points per thread where the plan
@@ -800,7 +806,7 @@ template <typename R, int N, bool W, bool INV>
 dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch,
                                hipStream_t st, RhsFlags done) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N, W>();
+  constexpr int T = static_cols<R, N, W, 1024>();
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const size_t lds = (size_t)N * lp * sizeof(C);
@@ -826,7 +832,7 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
 template <typename R, int N, bool W>
 dda_status launch_pass3_static_w(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N, W>();
+  constexpr int T = static_cols<R, N, W, 512>();
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const MvDims& d = op->dims;
