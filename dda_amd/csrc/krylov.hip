@@ -24,6 +24,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -1059,6 +1060,12 @@ struct Core {
     DDA_TRY_HIP(hipMemcpyAsync(host2, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));
     DDA_TRY_HIP(hipEventRecord(ev[0], st));
     int next_len = 2, chunk = 0;
+    // DDA_PROGRESS=<seconds>: a progress line on stderr at most that often, as
+    // the reference logs the residual during getPolVec
+    const char* pe = std::getenv("DDA_PROGRESS");
+    const double progress_s = pe ? std::atof(pe) : 0.0;
+    const auto t_start = std::chrono::steady_clock::now();
+    double last_print = 0;
     std::vector<double> prev_r(k, -1.0);
     std::vector<uint64_t> prev_count(k, 0);
     bool finished = false;
@@ -1097,6 +1104,16 @@ struct Core {
         len = std::max(len, li);
       }
       next_len = len;
+      if (progress_s > 0) {
+        const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (now - last_print >= progress_s) {
+          last_print = now;
+          int open = 0;
+          for (int i = 0; i < k; i++) open += hs[i].done ? 0 : 1;
+          std::fprintf(stderr, "dda: %.0f s, iteration %llu, |r|/|b| %.3e (rhs 0), %d of %d right-hand sides open\n",
+                       now, (unsigned long long)hs[0].count, std::sqrt(hs[0].residScale * hs[0].inprodR), open, k);
+        }
+      }
       chunk++;
     }
     (void)hipStreamSynchronize(st);
