@@ -39,6 +39,7 @@ struct MvDims {
   int S;       // vector stride
   int nrows;   // By * Bz
   int nmat;
+  int sorted;  // dipoles already in row order (row_dip is the identity)
 };
 
 // Per-right-hand-side stop flags of a device-side solver: the flag of RHS r is

@@ -449,7 +449,7 @@ __global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __rest
     const int c = g / d.nrows, row = g - c * d.nrows;
     const uint32_t s0 = row_start[row], s1 = row_start[row + 1];
     for (uint32_t q = s0 + t; q < s1; q += TPC) {
-      const uint32_t i = row_dip[q], meta = row_meta[q];
+      const uint32_t i = d.sorted ? q : row_dip[q], meta = row_meta[q];
       C v = x[(size_t)c * d.S + i];
       if (conj) v = cconj(v);
       tile[(meta & 0xffff) * lp + l] = cmul(ccs[3 * (meta >> 16) + c], v);
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __rest
   if (ok) {
     const uint32_t s0 = row_start[row], s1 = row_start[row + 1];
     for (uint32_t q = s0 + t; q < s1; q += TPC) {
-      const uint32_t i = row_dip[q], meta = row_meta[q];
+      const uint32_t i = d.sorted ? q : row_dip[q], meta = row_meta[q];
       C a = x[(size_t)c * d.S + i];
       if (conj) a = cconj(a);
       C r = cadd(cmul(ccs[3 * (meta >> 16) + c], tile[(meta & 0xffff) * lp + l]), a);
@@ -1484,6 +1484,8 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
     }
   }
   op->n_nonempty = (int)row_list.size();
+  d.sorted = 1;  // the reference's (z, y, x) dipole order makes the row index the identity
+  for (uint32_t k = 0; k < ds.nv && d.sorted; k++) d.sorted = row_dip[k] == k;
   op->static_z = has_static_plan(d.Gz) && !std::getenv("DDA_GENERIC_FFT");
   op->static_x = has_static_plan(d.Gx) && !std::getenv("DDA_GENERIC_FFT");
 
