@@ -225,6 +225,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default kernel variants)")
     ap.add_argument("--profile-rhs", type=int, default=1, help="with --matvec-profile: right-hand sides per launch")
     ap.add_argument("--large", action="store_true",
                     help="instead of the headline: the 1024^3 sphere (512 across) matvec rates and memory")
@@ -265,6 +266,7 @@ def main():
     ctx = D.Context(0)
     t0 = time.perf_counter()
     op = D.GpuMatVec(ctx, g.pos, g.mat, box, d, params.wave_num, precision=64, grid=grid)
+    tuning = op.tune(2) if not args.no_tune else "not tuned"  # kernel variants; FP32 twins follow
     t_setup = time.perf_counter() - t0
     modes = [m.strip() for m in args.runs.split(",") if m.strip()]
     if args.matvec_profile:
@@ -460,6 +462,7 @@ def main():
         "modes": modes_out,
         "mie": mie,
         "setup_s": t_setup,
+        "tuning": tuning,
         "device": info,
         "cpu_baseline": cpu,
         "serving": serving,
