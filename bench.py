@@ -266,7 +266,7 @@ def main():
     ctx = D.Context(0)
     t0 = time.perf_counter()
     op = D.GpuMatVec(ctx, g.pos, g.mat, box, d, params.wave_num, precision=64, grid=grid)
-    tuning = op.tune(2) if not args.no_tune else "not tuned"  # kernel variants; FP32 twins follow
+    tuning = op.tune(1) if not args.no_tune else "not tuned"  # kernel variants; FP32 twins follow
     t_setup = time.perf_counter() - t0
     modes = [m.strip() for m in args.runs.split(",") if m.strip()]
     if args.matvec_profile:
