@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--large-solves", action="store_true", help="with --large: also mixed solves to 1e-5")
     ap.add_argument("--large-solvers", default="bicgstab,qmr")
     ap.add_argument("--large-max-iter", type=int, default=20000)
+    ap.add_argument("--orient-dirs", type=int, default=8,
+                    help="directions of the extra orientation-batch measurement (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the seeded serving stream (0 = skip the serving measurement)")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
@@ -414,6 +416,29 @@ def main():
         except Exception as e:
             cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": "failed: %s" % e}
 
+    orient = None
+    if args.orient_dirs > 0:
+        try:
+            dirs = P.directions(args.orient_dirs, 2024)
+            einc_b, ccs_b, props_b, pols_b = P.incident_batch(params, dirs)
+            k = len(ccs_b)
+            op.set_couple_constants_batch(ccs_b)
+            sb = D.GpuBicgStab(op, max_iter=20000, mixed=True)
+            sb.get_pol_vecs(einc_b, args.eps)  # warm-up
+            t0 = time.perf_counter()
+            pb, infos_b = sb.get_pol_vecs(einc_b, args.eps)
+            el = time.perf_counter() - t0
+            sb.close()
+            base = modes_out.get("fp64:qmr", {}).get("solves_per_s")
+            orient = {"workload": "the same sphere, %d seeded directions x 2 polarisations = %d right-hand sides in "
+                                  "one block, mixed BiCGStab to %g" % (args.orient_dirs, k, args.eps),
+                      "solves_per_s": k / el, "seconds": el,
+                      "vs_fp64_qmr": (k / el) / base if base else None,
+                      "iterations": [int(i["iterations"]) for i in infos_b],
+                      "ok": sum(1 for i in infos_b if i["status"] == D.OK)}
+        except Exception as e:
+            orient = {"error": str(e)}
+
     serving = None
     if args.serve_jobs > 0:
         try:
@@ -465,6 +490,7 @@ def main():
         "tuning": tuning,
         "device": info,
         "cpu_baseline": cpu,
+        "orientation_batch": orient,
         "serving": serving,
     }
     print(json.dumps(line))
