@@ -43,7 +43,8 @@ EXPORTS = [
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
     "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_incident_plane", "dda_cross_sections",
-    "dda_far_field", "dda_jones",
+    "dda_far_field", "dda_jones", "dda_op_create_pack", "dda_op_pack_info", "dda_op_pack_map",
+    "dda_op_set_rhs_members",
 ]
 POL_CM, POL_LDR, POL_LDR_AVGPOL = 0, 1, 2
 POL_KINDS = {"cm": POL_CM, "ldr": POL_LDR, "ldr-avgpol": POL_LDR_AVGPOL}
@@ -158,6 +159,10 @@ def lib():
         L.dda_cross_sections.argtypes = [vp, vp, vp, i32, dp, dp, dp]
         L.dda_far_field.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int), dp, i32, dp]
         L.dda_jones.argtypes = [vp, vp, i32, dp, dp, dp, u32, dbl, dbl, u32, dbl, dbl, dp]
+        L.dda_op_create_pack.argtypes = [vp, ctypes.POINTER(OpDesc), i32, i32, vp, pp]
+        L.dda_op_pack_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(u32), ctypes.POINTER(u32)]
+        L.dda_op_pack_map.argtypes = [vp, i32, ctypes.POINTER(u32)]
+        L.dda_op_set_rhs_members.argtypes = [vp, ctypes.POINTER(i32), i32]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("dda_last_error", "dda_fft_fit"):
@@ -396,6 +401,93 @@ class GpuMatVec:
             self.close()
         except Exception:
             pass
+
+
+class PackMatVec(GpuMatVec):
+    """Several particles on one grid in one operator (dda_op_create_pack).
+
+    `members` is a list of (pos, mat, box, wave_num, nmat) sharing `grid_unit`
+    and `grid`. The operator works on the union of the members' positions;
+    `scatter` / `gather` move a member's [3, vec_stride_m] vector in and out of
+    the union layout, and `members_of` assigns each right-hand side of a batch
+    to a member. Couple constants are per right-hand side over the union's
+    materials (the last one is void and forced to 0): `pad_cc` widens a
+    member's [nmat_m, 3] set.
+    """
+
+    def __init__(self, ctx, members, grid_unit, precision=64, grid=(0, 0, 0), cache=None):
+        self.ctx = ctx
+        self.precision = precision
+        self._keep = []
+        descs = (OpDesc * len(members))()
+        for d, (pos, mat, box, wave_num, nmat) in zip(descs, members):
+            pos = np.ascontiguousarray(pos, np.uint32).reshape(-1, 3)
+            mat = np.ascontiguousarray(mat if mat is not None else np.zeros(len(pos)), np.uint8)
+            self._keep += [pos, mat]
+            d.box[:] = [int(b) for b in box]
+            d.grid[:] = [int(g) for g in grid]
+            d.nv = len(pos)
+            d.pos = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            d.mat = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            d.nmat = int(nmat if nmat is not None else int(mat.max()) + 1)
+            d.grid_unit = float(grid_unit)
+            d.wave_num = float(wave_num)
+        h = ctypes.c_void_p()
+        _check(lib().dda_op_create_pack(ctx.handle, descs, len(members), precision,
+                                        cache.handle if cache is not None else None, ctypes.byref(h)),
+               "dda_op_create_pack")
+        self.handle = h
+        ctx._adopt(self)
+        nm, nvu, nmu = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().dda_op_pack_info(h, ctypes.byref(nm), ctypes.byref(nvu), ctypes.byref(nmu)))
+        self.nmember, self.nv, self.nmat = int(nm.value), int(nvu.value), int(nmu.value)
+        self.member_nv = [int(d.nv) for d in descs]
+        self.member_nmat = [int(d.nmat) for d in descs]
+        self.maps = []
+        for m in range(self.nmember):
+            idx = np.zeros(self.member_nv[m], np.uint32)
+            _check(lib().dda_op_pack_map(h, m, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+            self.maps.append(idx.astype(np.int64))
+        self._keep = []
+        g = (ctypes.c_uint32 * 3)()
+        vs = ctypes.c_uint32()
+        nb = ctypes.c_size_t()
+        _check(lib().dda_op_info(h, g, ctypes.byref(vs), ctypes.byref(nb)))
+        self.grid = tuple(int(v) for v in g)
+        self.vec_stride = int(vs.value)
+        self.device_bytes = int(nb.value)
+        self.box = tuple(max(int(d.box[a]) for d in descs) for a in range(3))
+        self.grid_unit = float(grid_unit)
+        self.wave_nums = [float(d.wave_num) for d in descs]
+        self.wave_num = self.wave_nums[0]
+
+    def members_of(self, member):
+        """Member of each right-hand side (a sequence of member indices)."""
+        mm = np.ascontiguousarray(member, np.int32)
+        _check(lib().dda_op_set_rhs_members(self.handle, mm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(mm)),
+               "dda_op_set_rhs_members")
+
+    def pad_cc(self, member, cc_sqrt):
+        """A member's [nmat_m, 3] couple constants widened to the union's materials."""
+        out = np.zeros((self.nmat, 3), np.complex128)
+        cs = np.asarray(cc_sqrt, np.complex128).reshape(-1, 3)
+        out[:len(cs)] = cs
+        return out
+
+    def scatter(self, member, v):
+        """A member vector [3, >= nv_m] -> union layout [3, vec_stride] (zero elsewhere)."""
+        v = np.asarray(v).reshape(3, -1)
+        out = np.zeros((3, self.vec_stride), np.result_type(v.dtype, np.complex64))
+        out[:, self.maps[member]] = v[:, :self.member_nv[member]]
+        return out
+
+    def gather(self, member, u, stride=None):
+        """Union vector [3, vec_stride] -> the member's [3, stride] (padding zero)."""
+        u = np.asarray(u).reshape(3, -1)
+        n = self.member_nv[member]
+        out = np.zeros((3, stride or n), u.dtype)
+        out[:, :n] = u[:, self.maps[member]]
+        return out
 
 
 class MatVecGpu:

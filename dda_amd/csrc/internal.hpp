@@ -40,6 +40,11 @@ struct MvDims {
   int nrows;   // By * Bz
   int nmat;
   int sorted;  // dipoles already in row order (row_dip is the identity)
+  // Packs of several particles on one grid (nullptr otherwise): member[rhs]
+  // selects the material map (row_meta / mat at member * nv) and the tensor
+  // dhat_tab[member] of each right-hand side.
+  const int* member;
+  const void* const* dhat_tab;
 };
 
 // Per-right-hand-side stop flags of a device-side solver: the flag of RHS r is
@@ -141,6 +146,19 @@ struct dda_op_s {
   } tune;
   bool tuned = false;
   struct dda_dhat_cache_s* dhat_cache = nullptr;  // set when dhat is a cache entry (not owned)
+  // A pack of several particles on one grid (dda_op_create_pack): the union
+  // of their positions, each member's materials on it (void = nmat - 1,
+  // polarisability 0) and each member's tensor (its kd).
+  struct Pack {
+    int nmember = 0;
+    std::vector<uint8_t> mats;                 // [nmember][nv]
+    std::vector<double> wave_num;              // [nmember]
+    std::vector<std::vector<uint32_t>> index;  // member dipole -> union index
+    std::vector<void*> dhat;                   // [nmember], [0] = dhat
+    void** dhat_tab = nullptr;                 // device copy of dhat
+    int* member_dev = nullptr;                 // [cap_rhs] member of each right-hand side
+    std::vector<int> member_host;
+  } pack;
   void* pos_dev = nullptr;      // [nv][3] positions, uploaded on first use by the observables
   void* obs_scratch = nullptr;  // observables' work space
   size_t obs_bytes = 0;  // dda_op_tune ran (an FP32 twin of a tuned operator is tuned too)

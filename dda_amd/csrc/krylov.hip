@@ -259,13 +259,15 @@ __global__ __launch_bounds__(kRedThreads) void k_qmr_upd(typename CxT<R>::T* p_d
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_setup_b(typename CxT<R>::T* b, const double2* __restrict__ einc,
                                                          const typename CxT<R>::T* __restrict__ ccs,
-                                                         const uint8_t* __restrict__ mat, int nv, int S, int nmat) {
+                                                         const uint8_t* __restrict__ mat, int nv, int S, int nmat,
+                                                         const int* member) {
   typedef typename CxT<R>::T C;
   const long long n = 3ll * S;
   const int rhs = blockIdx.y;
   b += rhs * n;
   einc += rhs * n;
   ccs += rhs * 3 * nmat;
+  if (member) mat += (size_t)member[rhs] * nv;
   DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     C o = czero<C>();
@@ -278,13 +280,14 @@ __global__ __launch_bounds__(kRedThreads) void k_setup_b(typename CxT<R>::T* b, 
 template <typename R>
 __global__ void k_setup_start(typename CxT<R>::T* x, const double2* __restrict__ start,
                               const typename CxT<R>::T* __restrict__ ccs, const uint8_t* __restrict__ mat, int nv,
-                              int S, int nmat, double scale) {
+                              int S, int nmat, double scale, const int* member) {
   typedef typename CxT<R>::T C;
   const long long n = 3ll * S;
   const int rhs = blockIdx.y;
   x += rhs * n;
   start += rhs * n;
   ccs += rhs * 3 * nmat;
+  if (member) mat += (size_t)member[rhs] * nv;
   DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     double2 o = make_double2(0, 0);
@@ -300,13 +303,14 @@ __global__ void k_setup_start(typename CxT<R>::T* x, const double2* __restrict__
 template <typename R>
 __global__ void k_get_result(double2* out, const typename CxT<R>::T* __restrict__ x,
                              const typename CxT<R>::T* __restrict__ ccs, const uint8_t* __restrict__ mat, int nv, int S,
-                             int nmat, double scale) {
+                             int nmat, double scale, const int* member) {
   typedef typename CxT<R>::T C;
   const long long n = 3ll * S;
   const int rhs = blockIdx.y;
   out += rhs * n;
   x += rhs * n;
   ccs += rhs * 3 * nmat;
+  if (member) mat += (size_t)member[rhs] * nv;
   DDA_GRID_LOOP(t) {
     const int c = (int)(t / S), i = (int)(t - (long long)c * S);
     double2 o = make_double2(0, 0);
@@ -1238,16 +1242,16 @@ dda_status run_fp64(dda_solver_s* s, int k, double eps, bool have_start, dda_sol
   const double2* ccs = (const double2*)op->ccs;
   const double gu = op->grid_unit;
   const dim3 G(red_blocks(s->n), k), B(kRedThreads);
-  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, K.v[0], s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat);
+  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, K.v[0], s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat, d.member);
   if (have_start)
     hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, K.v[2], s->io + (size_t)s->cap * s->n, ccs, op->mat_dev,
-                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu));
+                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu), d.member);
   DDA_TRY_HIP(hipGetLastError());
   std::vector<SolverState> fin(k);
   std::vector<double> epsv(k, eps);
   DDA_TRY(K.solve(k, epsv.data(), nullptr, have_start, fin.data()));
   hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, K.v[2], ccs, op->mat_dev, d.nv, d.S, d.nmat,
-                     gu * gu * gu);
+                     gu * gu * gu, d.member);
   DDA_TRY_HIP(hipGetLastError());
   std::vector<double> rr;
   std::vector<int> none(k, 0);
@@ -1288,7 +1292,7 @@ dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_so
   const int nb = red_blocks(s->n);
   const dim3 G(nb, k), B(kRedThreads);
   const bool lanczos = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS;
-  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->b64, s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat);
+  hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->b64, s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat, d.member);
   hipLaunchKernelGGL(k_norm64, G, B, 0, st, s->b64, s->n, s->part64);
   DDA_TRY_HIP(hipGetLastError());
   std::vector<double> temp, rho;
@@ -1301,7 +1305,7 @@ dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_so
     }
   if (have_start)
     hipLaunchKernelGGL(k_setup_start<double>, G, B, 0, st, s->x64, s->io + (size_t)s->cap * s->n, ccs, op->mat_dev,
-                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu));
+                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu), d.member);
   else
     DDA_TRY_HIP(hipMemsetAsync(s->x64, 0, (size_t)k * s->n * sizeof(double2), st));
   s->outer_hist.clear();
@@ -1395,7 +1399,7 @@ dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_so
   }
   s->last_iters = iters[0];
   hipLaunchKernelGGL(k_get_result<double>, G, B, 0, st, s->io, s->x64, ccs, op->mat_dev, d.nv, d.S, d.nmat,
-                     gu * gu * gu);
+                     gu * gu * gu, d.member);
   DDA_TRY_HIP(hipGetLastError());
   dda_status worst = DDA_OK;
   for (int r = 0; r < k; r++) {
@@ -1500,6 +1504,7 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
   DDA_TRY(alloc_buffers(s, nrhs));
   if (s->mode == DDA_MODE_MIXED) {
     DDA_TRY(dda_op_set_cc_sqrt_batch(s->op32, op->cc_sqrt_host.data(), nrhs));
+    if (op->pack.nmember) DDA_TRY(dda_op_set_rhs_members(s->op32, op->pack.member_host.data(), nrhs));
     s->op32->prof = op->prof;
   }
   const size_t bytes = (size_t)nrhs * s->n * sizeof(double2);

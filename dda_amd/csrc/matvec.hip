@@ -57,6 +57,7 @@ __global__ __launch_bounds__(256) void k_pass1(const typename CxT<R>::T* __restr
   x += (long long)rhs * 3 * d.S;
   a1 += (long long)rhs * 3 * d.nrows * d.Gx;
   ccs += rhs * 3 * d.nmat;
+  if (d.member) row_meta += (size_t)d.member[rhs] * d.nv;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(256) void k_pass5(const typename CxT<R>::T* __restr
   x += (long long)rhs * 3 * d.S;
   y += (long long)rhs * 3 * d.S;
   ccs += rhs * 3 * d.nmat;
+  if (d.member) row_meta += (size_t)d.member[rhs] * d.nv;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int L = 3 * lr_n;
@@ -233,6 +235,7 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
   const int rhs = q % nrhs, grp = q / nrhs;
   if (stopped(fl, rhs)) return;
   a2 += (long long)rhs * 3 * d.Bz * d.Gy * d.Gx;
+  if (d.dhat_tab) dhat = (const typename CxT<R>::T*)d.dhat_tab[d.member[rhs]];
   const int ft = grp % ftn, fyp = grp / ftn;
   int ky = fyp, tile_x = ft;
   if (sub & 2) {
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_pass3s(type
   const int rhs = q % nrhs, grp = q / nrhs;
   if (stopped(fl, rhs)) return;
   a2 += (long long)rhs * 3 * d.Bz * d.Gy * d.Gx;
+  if (d.dhat_tab) dhat = (const typename CxT<R>::T*)d.dhat_tab[d.member[rhs]];
   const int ft = grp % ftn, fyp = grp / ftn;
   int ky = fyp, tile_x = ft;
   if (sub & 2) {
@@ -438,6 +442,7 @@ __global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __rest
   x += (long long)rhs * 3 * d.S;
   a1 += (long long)rhs * 3 * d.nrows * d.Gx;
   ccs += rhs * 3 * d.nmat;
+  if (d.member) row_meta += (size_t)d.member[rhs] * d.nv;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int t = threadIdx.x % TPC, l = threadIdx.x / TPC;
@@ -495,6 +500,7 @@ __global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __rest
   x += (long long)rhs * 3 * d.S;
   y += (long long)rhs * 3 * d.S;
   ccs += rhs * 3 * d.nmat;
+  if (d.member) row_meta += (size_t)d.member[rhs] * d.nv;
   extern __shared__ char smem[];
   C* tile = reinterpret_cast<C*>(smem);
   const int t = threadIdx.x % TPC, l = threadIdx.x / TPC;
@@ -1230,7 +1236,7 @@ static void cache_evict(dda_dhat_cache_s* c, size_t need) {
 static thread_local bool t_cache_hit = false;
 bool dda::last_tensor_cache_hit() { return t_cache_hit; }
 
-static dda_status cache_acquire(dda_dhat_cache_s* c, dda_op_s* op) {
+static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
   const MvDims& d = op->dims;
   const double kd = op->wave_num * op->grid_unit;
   const auto key = cache_key(d, kd, op->prec);
@@ -1247,8 +1253,7 @@ static dda_status cache_acquire(dda_dhat_cache_s* c, dda_op_s* op) {
     it->second.refs++;
     it->second.last_use = ++c->clock;
     c->hits++;
-    op->dhat = it->second.ptr;
-    op->dhat_cache = c;
+    *out = it->second.ptr;
     return DDA_OK;
   }
   c->misses++;
@@ -1294,7 +1299,12 @@ static dda_status cache_acquire(dda_dhat_cache_s* c, dda_op_s* op) {
   en.last_use = ++c->clock;
   c->bytes += bytes;
   c->cv.notify_all();
-  op->dhat = en.ptr;
+  *out = en.ptr;
+  return DDA_OK;
+}
+
+static dda_status cache_acquire(dda_dhat_cache_s* c, dda_op_s* op) {
+  DDA_TRY(cache_get(c, op, &op->dhat));
   op->dhat_cache = c;
   return DDA_OK;
 }
@@ -1391,10 +1401,22 @@ static void free_op(dda_op_s* op) {
     (void)hipEventDestroy(r.e);
   }
   for (auto e : op->prof_pool) (void)hipEventDestroy(e);
+  for (size_t m = 1; m < op->pack.dhat.size(); m++) {
+    void* p = op->pack.dhat[m];
+    if (!p) continue;
+    if (op->dhat_cache) {
+      cache_release(op->dhat_cache, p);
+    } else if (p != op->dhat &&
+               std::find(op->pack.dhat.begin() + 1, op->pack.dhat.begin() + m, p) == op->pack.dhat.begin() + m) {
+      (void)hipFree(p);  // owned; members of equal wave number share one
+    }
+  }
   if (op->dhat_cache) {
     cache_release(op->dhat_cache, op->dhat);
     op->dhat = nullptr;
   }
+  if (op->pack.dhat_tab) (void)hipFree(op->pack.dhat_tab);
+  if (op->pack.member_dev) (void)hipFree(op->pack.member_dev);
   void* bufs[] = {op->row_start, op->row_dip, op->row_meta, op->row_list, op->mat_dev, op->ccs, op->dhat, op->a1,
                   op->a2, op->tw[0], op->tw[1], op->tw[2], op->stage_x, op->stage_y, op->pos_dev, op->obs_scratch};
   for (void* b : bufs)
@@ -1614,6 +1636,226 @@ extern "C" dda_status dda_op_create_cached(dda_ctx_t ctx, const dda_op_desc* des
   return create_op(ctx, desc, precision, cache, nullptr, out);
 }
 
+
+// ------------------------------------------------------------------------------------
+// Packs: several particles on one grid in one operator. The operator runs on the
+// union of the members' dipole positions; right-hand side r belongs to member
+// member[r], whose materials (the void material, polarisability 0, where the
+// member has no dipole) and tensor (its wave number) the kernels select. A
+// vector of member m is zero off m's dipoles and stays so under the operator
+// (void rows are the identity), so a block solve over the pack solves each
+// member's system.
+// ------------------------------------------------------------------------------------
+
+static dda_status ensure_rhs_capacity(dda_op_s* op, int nrhs);
+
+// Gives `op` (created on the union with member 0's materials and tensor) the
+// other members' material maps and tensors. From `src` (an FP64 pack of the
+// same union) the tensors are converted, else taken from the cache or generated.
+static dda_status pack_extend(dda_op_s* op, const std::vector<uint8_t>& mats, const std::vector<double>& wave_num,
+                              const std::vector<std::vector<uint32_t>>& index, const dda_op_s* src) {
+  MvDims& d = op->dims;
+  const int nm = (int)wave_num.size();
+  hipStream_t st = op->ctx->stream;
+  op->pack.nmember = nm;
+  op->pack.mats = mats;
+  op->pack.wave_num = wave_num;
+  op->pack.index = index;
+  op->pack.dhat.assign(nm, nullptr);
+  op->pack.dhat[0] = op->dhat;
+  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
+  const size_t dn = 6ull * (d.Gx / 2 + 1) * (d.Gy / 2 + 1) * (d.Gz / 2 + 1);
+  const double w0 = op->wave_num;
+  for (int m = 1; m < nm; m++) {
+    if (!op->dhat_cache) {
+      int same = -1;
+      for (int k = 0; k < m && same < 0; k++)
+        if (wave_num[k] == wave_num[m]) same = k;
+      if (same >= 0) {
+        op->pack.dhat[m] = op->pack.dhat[same];
+        continue;
+      }
+      DDA_TRY_HIP(hipMalloc(&op->pack.dhat[m], dn * csz));
+      op->device_bytes += dn * csz;
+      if (src) {
+        hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, st, (const double2*)src->pack.dhat[m],
+                           (float2*)op->pack.dhat[m], (long long)dn);
+        DDA_TRY_HIP(hipGetLastError());
+        DDA_TRY_HIP(hipStreamSynchronize(st));
+        continue;
+      }
+    }
+    op->wave_num = wave_num[m];
+    const dda_status rc = op->dhat_cache ? cache_get(op->dhat_cache, op, &op->pack.dhat[m])
+                                         : generate_dhat(op, d.Bx, d.By, d.Bz, op->pack.dhat[m]);
+    op->wave_num = w0;
+    DDA_TRY(rc);
+  }
+  // materials and row metadata of every member ([nm][nv]); the union is in
+  // (z, y, x) order, so row k holds dipole k
+  std::vector<uint32_t> meta((size_t)nm * d.nv);
+  for (int m = 0; m < nm; m++)
+    for (int k = 0; k < d.nv; k++)
+      meta[(size_t)m * d.nv + k] = op->pos[3 * (size_t)k] | ((uint32_t)mats[(size_t)m * d.nv + k] << 16);
+  uint32_t* row_meta = nullptr;
+  uint8_t* mat_dev = nullptr;
+  DDA_TRY_HIP(hipMalloc(&row_meta, meta.size() * 4));
+  hipError_t e = hipMalloc(&mat_dev, mats.size());
+  if (e != hipSuccess) (void)hipFree(row_meta);
+  DDA_TRY_HIP(e);
+  (void)hipFree(op->row_meta);
+  (void)hipFree(op->mat_dev);
+  op->row_meta = row_meta;
+  op->mat_dev = mat_dev;
+  op->device_bytes += (size_t)(nm - 1) * d.nv * 5;
+  DDA_TRY_HIP(copy_sync(row_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, st));
+  DDA_TRY_HIP(copy_sync(mat_dev, mats.data(), mats.size(), hipMemcpyHostToDevice, st));
+  DDA_TRY_HIP(hipMalloc(&op->pack.dhat_tab, nm * sizeof(void*)));
+  DDA_TRY_HIP(copy_sync(op->pack.dhat_tab, op->pack.dhat.data(), nm * sizeof(void*), hipMemcpyHostToDevice, st));
+  op->pack.member_host.assign(op->cap_rhs, 0);
+  DDA_TRY_HIP(hipMalloc(&op->pack.member_dev, op->cap_rhs * sizeof(int)));
+  DDA_TRY_HIP(copy_sync(op->pack.member_dev, op->pack.member_host.data(), op->cap_rhs * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  d.member = op->pack.member_dev;
+  d.dhat_tab = op->pack.dhat_tab;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_create_pack(dda_ctx_t ctx, const dda_op_desc* descs, int nmember, int precision,
+                                         dda_dhat_cache_t cache, dda_op_t* out) {
+  if (!ctx || !descs || !out || nmember < 1 || nmember > 4096) {
+    set_error("dda_op_create_pack: null argument or nmember not in 1..4096");
+    return DDA_INVALID_ARG;
+  }
+  *out = nullptr;
+  uint32_t box[3] = {0, 0, 0};
+  uint32_t nmat = 0;
+  for (int m = 0; m < nmember; m++) {
+    const dda_op_desc& ds = descs[m];
+    if (ds.dmatrix || !ds.pos || ds.nv == 0 || ds.nmat == 0 || ds.nmat > 255 || ds.grid_unit != descs[0].grid_unit ||
+        std::memcmp(ds.grid, descs[0].grid, sizeof ds.grid) != 0 || !(ds.wave_num > 0)) {
+      set_error("dda_op_create_pack: members need positions, 1..255 materials, no host tensor, and one grid and "
+                "grid unit");
+      return DDA_INVALID_ARG;
+    }
+    for (int a = 0; a < 3; a++) box[a] = std::max(box[a], ds.box[a]);
+    nmat = std::max(nmat, ds.nmat + 1);
+  }
+  for (int a = 0; a < 3; a++)
+    if (box[a] == 0 || box[a] > 65535) {
+      set_error("dda_op_create_pack: box extents must be in 1..65535");
+      return DDA_INVALID_ARG;
+    }
+  // the union of the positions in (z, y, x) order
+  auto key = [&](const uint32_t* p) {
+    return ((uint64_t)p[2] * box[1] + p[1]) * box[0] + p[0];
+  };
+  std::vector<uint64_t> keys;
+  for (int m = 0; m < nmember; m++) {
+    const dda_op_desc& ds = descs[m];
+    for (uint32_t i = 0; i < ds.nv; i++) {
+      const uint32_t* p = ds.pos + 3 * (size_t)i;
+      if (p[0] >= ds.box[0] || p[1] >= ds.box[1] || p[2] >= ds.box[2] || (ds.mat && ds.mat[i] >= ds.nmat)) {
+        set_error("dda_op_create_pack: a dipole outside its box or a material index >= nmat");
+        return DDA_INVALID_ARG;
+      }
+      keys.push_back(key(p));
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const size_t nu = keys.size();
+  if (nu > 0xffffffffu) {
+    set_error("dda_op_create_pack: union too large");
+    return DDA_INVALID_ARG;
+  }
+  std::vector<uint32_t> pos(3 * nu);
+  for (size_t k = 0; k < nu; k++) {
+    pos[3 * k] = (uint32_t)(keys[k] % box[0]);
+    pos[3 * k + 1] = (uint32_t)(keys[k] / box[0] % box[1]);
+    pos[3 * k + 2] = (uint32_t)(keys[k] / box[0] / box[1]);
+  }
+  const uint8_t vd = (uint8_t)(nmat - 1);
+  std::vector<uint8_t> mats((size_t)nmember * nu, vd);
+  std::vector<std::vector<uint32_t>> index(nmember);
+  std::vector<double> wave_num(nmember);
+  for (int m = 0; m < nmember; m++) {
+    const dda_op_desc& ds = descs[m];
+    wave_num[m] = ds.wave_num;
+    index[m].resize(ds.nv);
+    uint8_t* mm = &mats[(size_t)m * nu];
+    for (uint32_t i = 0; i < ds.nv; i++) {
+      const uint64_t k = key(ds.pos + 3 * (size_t)i);
+      const uint32_t u = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), k) - keys.begin());
+      if (mm[u] != vd) {
+        set_error("dda_op_create_pack: two dipoles of one member at the same position");
+        return DDA_INVALID_ARG;
+      }
+      index[m][i] = u;
+      mm[u] = ds.mat ? ds.mat[i] : 0;
+    }
+  }
+  dda_op_desc ud;
+  std::memset(&ud, 0, sizeof ud);
+  for (int a = 0; a < 3; a++) {
+    ud.box[a] = box[a];
+    ud.grid[a] = descs[0].grid[a];
+  }
+  ud.nv = (uint32_t)nu;
+  ud.pos = pos.data();
+  ud.mat = mats.data();
+  ud.nmat = nmat;
+  ud.grid_unit = descs[0].grid_unit;
+  ud.wave_num = descs[0].wave_num;
+  dda_op_t op = nullptr;
+  DDA_TRY(create_op(ctx, &ud, precision, cache, nullptr, &op));
+  const dda_status rc = pack_extend(op, mats, wave_num, index, nullptr);
+  if (rc != DDA_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_pack_info(dda_op_t op, int* nmember, uint32_t* nv_union, uint32_t* nmat_union) {
+  if (!op) {
+    set_error("dda_op_pack_info: null operator");
+    return DDA_INVALID_ARG;
+  }
+  if (nmember) *nmember = op->pack.nmember;
+  if (nv_union) *nv_union = (uint32_t)op->dims.nv;
+  if (nmat_union) *nmat_union = op->nmat;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_pack_map(dda_op_t op, int member, uint32_t* index) {
+  if (!op || !index || member < 0 || member >= op->pack.nmember) {
+    set_error("dda_op_pack_map: null argument, not a pack, or member out of range");
+    return DDA_INVALID_ARG;
+  }
+  std::copy(op->pack.index[member].begin(), op->pack.index[member].end(), index);
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_set_rhs_members(dda_op_t op, const int* member, int nrhs) {
+  if (!op || !member || nrhs < 1 || op->pack.nmember == 0) {
+    set_error("dda_op_set_rhs_members: null argument, nrhs < 1, or not a pack");
+    return DDA_INVALID_ARG;
+  }
+  for (int r = 0; r < nrhs; r++)
+    if (member[r] < 0 || member[r] >= op->pack.nmember) {
+      set_error("dda_op_set_rhs_members: member index out of range");
+      return DDA_INVALID_ARG;
+    }
+  DDA_TRY(bind_device(op->ctx));
+  DDA_TRY(ensure_rhs_capacity(op, nrhs));
+  std::copy(member, member + nrhs, op->pack.member_host.begin());
+  DDA_TRY_HIP(copy_sync(op->pack.member_dev, op->pack.member_host.data(), nrhs * sizeof(int), hipMemcpyHostToDevice,
+                        op->ctx->stream));
+  return DDA_OK;
+}
+
 // The FP32 twin of an FP64 operator (same geometry; tensor converted from the
 // FP64 one, or from the same cache).
 dda_status dda::create_twin32(const dda_op_s* op, dda_op_t* out) {
@@ -1633,7 +1875,16 @@ dda_status dda::create_twin32(const dda_op_s* op, dda_op_t* out) {
   desc.nmat = op->nmat;
   desc.grid_unit = op->grid_unit;
   desc.wave_num = op->wave_num;
-  return create_op(op->ctx, &desc, 32, op->dhat_cache, op, out);
+  DDA_TRY(create_op(op->ctx, &desc, 32, op->dhat_cache, op, out));
+  if (op->pack.nmember) {
+    dda_status rc = pack_extend(*out, op->pack.mats, op->pack.wave_num, op->pack.index, op);
+    if (rc != DDA_OK) {
+      free_op(*out);
+      *out = nullptr;
+      return rc;
+    }
+  }
+  return DDA_OK;
 }
 
 extern "C" dda_status dda_op_destroy(dda_op_t op) {
@@ -1685,6 +1936,16 @@ static dda_status ensure_rhs_capacity(dda_op_s* op, int nrhs) {
   op->a1 = a1;
   op->a2 = a2;
   op->ccs = ccs;
+  if (op->pack.nmember) {
+    int* mem = nullptr;
+    DDA_TRY_HIP(hipMalloc(&mem, nrhs * sizeof(int)));
+    op->pack.member_host.resize(nrhs, 0);
+    DDA_TRY_HIP(copy_sync(mem, op->pack.member_host.data(), nrhs * sizeof(int), hipMemcpyHostToDevice,
+                          op->ctx->stream));
+    (void)hipFree(op->pack.member_dev);
+    op->pack.member_dev = mem;
+    op->dims.member = mem;
+  }
   op->cap_rhs = nrhs;
   return DDA_OK;
 }
@@ -1695,15 +1956,19 @@ static dda_status upload_ccs(dda_op_s* op, const double* cc_sqrt, int first, int
   const double f = std::pow(op->grid_unit, -1.5);
   const size_t n = 3ull * op->nmat * nrhs;
   const size_t off = 3ull * op->nmat * first;
+  // the last material of a pack is the void one: no polarisability
+  const size_t nm3 = 3ull * op->nmat;
+  auto keep = [&](size_t i) { return op->pack.nmember == 0 || i % nm3 < nm3 - 3 ? 1.0 : 0.0; };
   hipError_t e;
   if (op->prec == 64) {
     std::vector<double2> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = make_double2(cc_sqrt[2 * i] * f, cc_sqrt[2 * i + 1] * f);
+    for (size_t i = 0; i < n; i++) h[i] = make_double2(cc_sqrt[2 * i] * f * keep(i), cc_sqrt[2 * i + 1] * f * keep(i));
     e = hipMemcpyAsync((double2*)op->ccs + off, h.data(), n * sizeof(double2), hipMemcpyHostToDevice, op->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
   } else {
     std::vector<float2> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = make_float2((float)(cc_sqrt[2 * i] * f), (float)(cc_sqrt[2 * i + 1] * f));
+    for (size_t i = 0; i < n; i++)
+      h[i] = make_float2((float)(cc_sqrt[2 * i] * f * keep(i)), (float)(cc_sqrt[2 * i + 1] * f * keep(i)));
     e = hipMemcpyAsync((float2*)op->ccs + off, h.data(), n * sizeof(float2), hipMemcpyHostToDevice, op->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
   }

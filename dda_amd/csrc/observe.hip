@@ -60,8 +60,9 @@ __global__ __launch_bounds__(kObsThreads) void k_cross(const double2* __restrict
                                                        const double2* __restrict__ einc,
                                                        const uint8_t* __restrict__ mat,
                                                        const double* __restrict__ mdr, int nv, int S, int nmat,
-                                                       double* part) {
+                                                       double* part, const int* member) {
   const int r = blockIdx.y;
+  if (member) mat += (size_t)member[r] * nv;
   const double2* p = pol + (long long)r * 3 * S;
   const double2* e = einc + (long long)r * 3 * S;
   const double* md = mdr + (long long)r * 3 * nmat;
@@ -229,11 +230,26 @@ extern "C" dda_status dda_couple_constants(int kind, uint32_t nmat, const double
   return DDA_OK;
 }
 
+// Observables scale by the operator's wave number: a pack qualifies only when
+// its members share one.
+static dda_status one_wave_number(const dda_op_s* op, const char* fn) {
+  for (double w : op->pack.wave_num)
+    if (w != op->wave_num) {
+      set_error(std::string(fn) + ": a pack of members with different wave numbers");
+      return DDA_NOT_SUPPORTED;
+    }
+  return DDA_OK;
+}
+
 extern "C" dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* prop, const double* pol,
                                          void* einc_dev) {
   if (!op || nrhs < 1 || !prop || !pol || !einc_dev) {
     set_error("dda_incident_plane: null argument or nrhs < 1");
     return DDA_INVALID_ARG;
+  }
+  if (op->pack.nmember > 1) {
+    set_error("dda_incident_plane: a pack (the field of member m is its own operator's, scattered by the pack map)");
+    return DDA_NOT_SUPPORTED;
   }
   DDA_TRY(bind_device(op->ctx));
   DDA_TRY(positions_on_device(op));
@@ -261,6 +277,7 @@ extern "C" dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const
     set_error("dda_cross_sections: null argument or nrhs < 1");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(one_wave_number(op, "dda_cross_sections"));
   DDA_TRY(bind_device(op->ctx));
   const MvDims& d = op->dims;
   const int nb = std::max(1, std::min(256, (d.nv + 4095) / 4096));
@@ -280,7 +297,7 @@ extern "C" dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const
   DDA_TRY_HIP(hipMemcpyAsync(mdr_dev, mdr.data(), mdr.size() * sizeof(double), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_cross, dim3(nb, nrhs), dim3(kObsThreads), 0, st, (const double2*)pol_dev,
                      (const double2*)(einc_dev ? einc_dev : pol_dev), (const uint8_t*)op->mat_dev, mdr_dev, d.nv, d.S,
-                     (int)op->nmat, part);
+                     (int)op->nmat, part, d.member);
   DDA_TRY_HIP(hipGetLastError());
   std::vector<double> h(2ull * nb * nrhs);
   DDA_TRY_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -303,6 +320,7 @@ extern "C" dda_status dda_far_field(dda_op_t op, const void* pol_dev, int nrhs, 
     set_error("dda_far_field: null argument, nrhs < 1 or no directions");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(one_wave_number(op, "dda_far_field"));
   DDA_TRY(bind_device(op->ctx));
   for (int i = 0; i < nq; i++)
     if (rhs && (rhs[i] < 0 || rhs[i] >= nrhs)) {
@@ -345,6 +363,7 @@ extern "C" dda_status dda_jones(dda_op_t op, const void* pol_dev, int symmetric,
     set_error("dda_jones: null argument or empty angle grid");
     return DDA_INVALID_ARG;
   }
+  DDA_TRY(one_wave_number(op, "dda_jones"));
   const size_t count = (size_t)ntheta * nphi;
   // two field evaluations per angle: (p1; X = pol2, Y = pol1) and
   // (p2; X = pol2, Y = pol1) or, for a symmetric target, (p1; X = pol1, Y = -pol2)

@@ -125,6 +125,29 @@ dda_status dda_dhat_cache_stats(dda_dhat_cache_t cache, uint64_t counts[3], size
  * twin of a mixed-mode solver over it uses the same cache. */
 dda_status dda_op_create_cached(dda_ctx_t ctx, const dda_op_desc* desc, int precision, dda_dhat_cache_t cache,
                                 dda_op_t* out);
+/* Packs (no counterpart in the reference, which runs one particle per
+ * operator): one operator for nmember particles on the same grid and grid unit
+ * (each descs[m] with its own box <= the union box, positions, materials and
+ * wave number; grid[] equal in all, 0 = fitted to the largest box). It works on
+ * the union of the members' positions in (z, y, x) order; its materials are
+ * 0 .. max(nmat_m) - 1 plus a last, void material (polarisability forced to 0)
+ * that marks the union positions where a member has no dipole. Every
+ * right-hand side belongs to one member (dda_op_set_rhs_members, default 0) and
+ * sees that member's materials and tensor; a vector that is zero off the
+ * member's dipoles stays so, so a block solve over the pack solves each
+ * member's own system. cache may be NULL (tensors generated, shared between
+ * members of equal wave number). Observables (dda_cross_sections ...) reject
+ * packs whose members differ in wave number. */
+dda_status dda_op_create_pack(dda_ctx_t ctx, const dda_op_desc* descs, int nmember, int precision,
+                              dda_dhat_cache_t cache, dda_op_t* out);
+/* Members, union dipole count and union material count (with the void one) of
+ * a pack; nmember = 0 for an operator that is not one. */
+dda_status dda_op_pack_info(dda_op_t op, int* nmember, uint32_t* nv_union, uint32_t* nmat_union);
+/* index[i] = union position of member `member`'s dipole i (descs[member].nv
+ * entries): a member's vector entry i lives at union entry index[i]. */
+dda_status dda_op_pack_map(dda_op_t op, int member, uint32_t* index);
+/* Member of each of the first nrhs right-hand sides of a pack. */
+dda_status dda_op_set_rhs_members(dda_op_t op, const int* member, int nrhs);
 /* Grid, vector stride and device bytes held by the operator. */
 dda_status dda_op_info(dda_op_t op, uint32_t grid[3], uint32_t* vec_stride, size_t* device_bytes);
 /* GpuMatVec::setCoupleConstants, DDA/GpuMatVec.cpp:139-142: sqrt of the
