@@ -204,11 +204,19 @@ def serving_bench(args, D):
         stats = srv.stats()
     finally:
         srv.close()
+    # the same burst with different particles of one grid packed into one solve
+    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
+    try:
+        S.run_stream(srv, S.make_stream(2, seed=7))
+        burst_pg = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=42))
+    finally:
+        srv.close()
     strip = lambda r: {k: v for k, v in r.items()}
     return {"workload": "%d jobs per run, sphere/box/ellipsoid, extent U{16..96}, Re m U(1.2,2), Im m U(0,0.2), "
                         "|m|kd=0.5, random orientation, 2 polarisations, mixed BiCGStab to %g, 4 slots"
                         % (args.serve_jobs, args.eps),
-            "capacity": strip(burst), "poisson_0.9x": dict(strip(stream), rate_jobs_per_s=rate),
+            "capacity": strip(burst), "capacity_pack_grids": strip(burst_pg),
+            "poisson_0.9x": dict(strip(stream), rate_jobs_per_s=rate),
             "p50_ms": stream["p50_ms"], "p99_ms": stream["p99_ms"], "solves_per_s": stream["solves_per_s"],
             "server": stats}
 

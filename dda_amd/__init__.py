@@ -80,7 +80,8 @@ class SolveInfo(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("cache_bytes", ctypes.c_size_t), ("solver_kind", ctypes.c_int), ("mode", ctypes.c_int),
-                ("eps", ctypes.c_double), ("max_iter", ctypes.c_uint64), ("max_pack", ctypes.c_int)]
+                ("eps", ctypes.c_double), ("max_iter", ctypes.c_uint64), ("max_pack", ctypes.c_int),
+                ("pack_grids", ctypes.c_int)]
 
 
 class JobDesc(ctypes.Structure):
@@ -651,11 +652,12 @@ class TensorCache:
 class Server:
     """Scattering-job server (dda_server_*): queue, HBM admission, `slots`
     concurrent block solves on their own streams, tensor cache, packing of
-    jobs of the same particle."""
+    jobs of the same particle (with pack_grids, of any particles on one grid)."""
 
     def __init__(self, device=0, slots=4, solver="bicgstab", mixed=True, eps=1e-6, max_iter=0, max_pack=16,
-                 hbm_budget=0, cache_bytes=0):
+                 hbm_budget=0, cache_bytes=0, pack_grids=False):
         c = ServerConfig()
+        c.pack_grids = int(bool(pack_grids))
         c.device, c.slots, c.hbm_budget, c.cache_bytes = device, slots, int(hbm_budget), int(cache_bytes)
         c.solver_kind = SOLVER_KINDS[solver]
         c.mode = MODE_MIXED if mixed else MODE_FP64

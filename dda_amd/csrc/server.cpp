@@ -2,7 +2,8 @@
 // footprint, worker slots that each own a context (one HIP stream) so that
 // admitted jobs run concurrently on the device, a shared device cache of FFT'd
 // interaction tensors keyed by (grid, kd), and packing of queued jobs of the
-// same particle into one block solve. No counterpart in the reference, which
+// same particle (optionally: of any particle on the same grid) into one block
+// solve. No counterpart in the reference, which
 // runs one geometry per process (DDA/DDAMain.cpp:550-663).
 //
 // Continuous batching: a slot that finishes takes the next admissible job at
@@ -57,6 +58,12 @@ uint64_t fnv(uint64_t h, const void* p, size_t n) {
   return h;
 }
 
+// Jobs that can share a pack operator (dda_op_create_pack): one grid and
+// grid unit, and room for the void material.
+bool same_grid(const Job& a, const Job& b) {
+  return a.grid_unit == b.grid_unit && std::equal(a.grid, a.grid + 3, b.grid) && a.nmat < 256 && b.nmat < 256;
+}
+
 bool same_particle(const Job& a, const Job& b) {
   return a.key == b.key && a.nv == b.nv && a.stride == b.stride && a.nmat == b.nmat &&
          a.grid_unit == b.grid_unit && a.wave_num == b.wave_num && std::equal(a.box, a.box + 3, b.box) &&
@@ -99,11 +106,136 @@ struct dda_server_s {
     return b;
   }
 
+  // Footprint of a pack of distinct particles: the union box and at most
+  // the sum of their dipoles.
+  size_t footprint(const std::vector<const Job*>& parts, int k) const {
+    if (parts.size() == 1) return footprint(*parts[0], k);
+    Job u;
+    for (const Job* p : parts) {
+      for (int a = 0; a < 3; a++) {
+        u.box[a] = std::max(u.box[a], p->box[a]);
+        u.grid[a] = p->grid[a];
+      }
+      u.nv += p->nv;
+    }
+    u.stride = (u.nv + 15) / 16 * 16;
+    return footprint(u, k) + parts.size() * 5 * u.nv;
+  }
+
   void run_pack(dda_ctx_t ctx, std::vector<Job*>& pack);
+  void run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::vector<const Job*>& parts);
   void worker(int slot);
 };
 
+// Distinct particles of one grid: a pack operator over the union of their
+// positions; each job's fields scattered into the union layout and its
+// polarisations gathered back.
+void dda_server_s::run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::vector<const Job*>& parts) {
+  const int nm = (int)parts.size();
+  std::vector<dda_op_desc> descs(nm);
+  for (int m = 0; m < nm; m++) {
+    const Job& p = *parts[m];
+    dda_op_desc& d = descs[m];
+    std::memset(&d, 0, sizeof d);
+    std::copy(p.box, p.box + 3, d.box);
+    std::copy(p.grid, p.grid + 3, d.grid);
+    d.nv = p.nv;
+    d.pos = p.pos.data();
+    d.mat = p.mat.data();
+    d.nmat = p.nmat;
+    d.grid_unit = p.grid_unit;
+    d.wave_num = p.wave_num;
+  }
+  int k = 0;
+  for (Job* j : pack) k += j->nrhs;
+  dda_op_t op = nullptr;
+  dda_solver_t s = nullptr;
+  const double t0 = now();
+  dda_status rc = dda_op_create_pack(ctx, descs.data(), nm, 64, cache, &op);
+  const bool hit = rc == DDA_OK && dda::last_tensor_cache_hit();
+  int nmember = 0;
+  uint32_t nvu = 0, nmu = 0, S = 0;
+  if (rc == DDA_OK) rc = dda_op_pack_info(op, &nmember, &nvu, &nmu);
+  if (rc == DDA_OK) rc = dda_op_info(op, nullptr, &S, nullptr);
+  std::vector<std::vector<uint32_t>> maps(nm);
+  for (int m = 0; m < nm && rc == DDA_OK; m++) {
+    maps[m].resize(parts[m]->nv);
+    rc = dda_op_pack_map(op, m, maps[m].data());
+  }
+  const size_t n = 3ull * S;
+  std::vector<double> cc(2ull * 3 * nmu * k, 0.0), einc(2 * n * k, 0.0), pol(2 * n * k);
+  std::vector<int> member(k);
+  std::vector<dda_solve_info> info(k);
+  std::vector<int> job_member(pack.size());
+  if (rc == DDA_OK) {
+    int r0 = 0;
+    for (size_t q = 0; q < pack.size(); q++) {
+      const Job& j = *pack[q];
+      int m = 0;
+      while (!same_particle(j, *parts[m])) m++;
+      job_member[q] = m;
+      for (int r = 0; r < j.nrhs; r++, r0++) {
+        member[r0] = m;
+        std::copy(j.cc.begin() + 6ull * j.nmat * r, j.cc.begin() + 6ull * j.nmat * (r + 1),
+                  cc.begin() + 6ull * nmu * r0);
+        for (int c = 0; c < 3; c++)
+          for (uint32_t i = 0; i < j.nv; i++) {
+            const size_t src = 2 * ((size_t)r * 3 * j.stride + (size_t)c * j.stride + i);
+            const size_t dst = 2 * ((size_t)r0 * n + (size_t)c * S + maps[m][i]);
+            einc[dst] = j.einc[src];
+            einc[dst + 1] = j.einc[src + 1];
+          }
+      }
+    }
+  }
+  if (rc == DDA_OK) rc = dda_op_set_cc_sqrt_batch(op, cc.data(), k);
+  if (rc == DDA_OK) rc = dda_op_set_rhs_members(op, member.data(), k);
+  if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
+  const double setup = now() - t0;
+  if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
+  const bool call_failed = rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED ||
+                           (rc == DDA_INVALID_ARG && !s);
+  const std::string err = rc == DDA_OK ? std::string() : std::string(dda_last_error());
+  if (s) dda_solver_destroy(s);
+  if (op) dda_op_destroy(op);
+  int r0 = 0;
+  for (size_t q = 0; q < pack.size(); q++) {
+    Job* j = pack[q];
+    const int m = job_member[q];
+    j->pol.assign(6ull * j->stride * j->nrhs, 0.0);
+    if (!call_failed)
+      for (int r = 0; r < j->nrhs; r++)
+        for (int c = 0; c < 3; c++)
+          for (uint32_t i = 0; i < j->nv; i++) {
+            const size_t src = 2 * ((size_t)(r0 + r) * n + (size_t)c * S + maps[m][i]);
+            const size_t dst = 2 * ((size_t)r * 3 * j->stride + (size_t)c * j->stride + i);
+            j->pol[dst] = pol[src];
+            j->pol[dst + 1] = pol[src + 1];
+          }
+    j->info.assign(info.begin() + r0, info.begin() + r0 + j->nrhs);
+    int st = DDA_OK;
+    for (int r = 0; r < j->nrhs; r++)
+      if (info[r0 + r].status != DDA_OK && st == DDA_OK) st = info[r0 + r].status;
+    j->res.status = call_failed ? (int)rc : st;
+    j->res.packed_rhs = k;
+    j->res.cache_hit = hit ? 1 : 0;
+    j->res.setup_s = setup;
+    j->res.iterations = 0;
+    for (int r = 0; r < j->nrhs; r++) j->res.iterations = std::max<int>(j->res.iterations, (int)info[r0 + r].iterations);
+    j->error = err;
+    r0 += j->nrhs;
+  }
+}
+
 void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
+  std::vector<const Job*> parts;
+  for (Job* j : pack)
+    if (std::none_of(parts.begin(), parts.end(), [&](const Job* p) { return same_particle(*p, *j); }))
+      parts.push_back(j);
+  if (parts.size() > 1) {
+    run_multi(ctx, pack, parts);
+    return;
+  }
   const Job& j0 = *pack[0];
   int k = 0;
   for (Job* j : pack) k += j->nrhs;
@@ -176,22 +308,38 @@ void dda_server_s::worker(int slot) {
         cv_work.wait(lk);
       }
       if (queue.empty()) break;
-      // the head job and queued jobs of the same particle, up to max_pack RHS
+      // the head job and queued jobs of the same particle (with pack_grids,
+      // of any particle on its grid), up to max_pack RHS
       Job* h = queue.front();
       queue.pop_front();
       pack.push_back(h);
       int k = h->nrhs;
+      std::vector<const Job*> parts{h};
+      const size_t cap = std::max(job_budget, footprint(*h, h->nrhs));
       for (auto it = queue.begin(); it != queue.end() && k < cfg.max_pack;) {
-        if (same_particle(**it, *h) && k + (*it)->nrhs <= cfg.max_pack &&
-            in_use + footprint(*h, k + (*it)->nrhs) <= std::max(job_budget, footprint(*h, h->nrhs))) {
-          k += (*it)->nrhs;
+        const Job& j = **it;
+        bool fits = k + j.nrhs <= cfg.max_pack;
+        bool is_new = false;
+        if (fits) {
+          is_new = std::none_of(parts.begin(), parts.end(), [&](const Job* p) { return same_particle(*p, j); });
+          fits = !is_new || (cfg.pack_grids && same_grid(j, *h) &&
+                             std::all_of(parts.begin(), parts.end(), [&](const Job* p) { return p->nmat < 256; }));
+        }
+        if (fits) {
+          std::vector<const Job*> np = parts;
+          if (is_new) np.push_back(&j);
+          fits = in_use + footprint(np, k + j.nrhs) <= cap;
+          if (fits) parts.swap(np);
+        }
+        if (fits) {
+          k += j.nrhs;
           pack.push_back(*it);
           it = queue.erase(it);
         } else {
           ++it;
         }
       }
-      foot = footprint(*h, k);
+      foot = footprint(parts, k);
       in_use += foot;
       peak = std::max(peak, in_use);
       const double t = now();

@@ -331,6 +331,9 @@ typedef struct dda_server_config {
   double eps;          /* relative residual */
   uint64_t max_iter;   /* 0 = 3 * box volume */
   int max_pack;        /* right-hand sides per block solve, >= 1 */
+  int pack_grids;      /* 1: also pack different particles on one grid and grid unit
+                          (dda_op_create_pack; a job's result then equals its solve
+                          alone to the solver's tolerance, not bit for bit) */
 } dda_server_config;
 typedef struct dda_job_desc {
   dda_op_desc geometry;  /* as for dda_op_create; dmatrix must be NULL */

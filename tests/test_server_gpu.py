@@ -117,3 +117,31 @@ def test_server_stream_overlaps():
     overlapping = sum(1 for a in spans for b in spans if a < b and a[0] < b[1] and b[0] < a[1])
     assert overlapping >= 1
     assert r["p99_ms"] >= r["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("solver,mixed", [("bicgstab", True), ("qmr", False)])
+def test_server_packs_particles_on_one_grid(solver, mixed):
+    """pack_grids: queued jobs of different particles on one grid run as one
+    block solve over a pack operator; each result equals the job's solve alone
+    to the solver's tolerance."""
+    rng = np.random.default_rng(21)
+    jobs = [S.make_job(rng, "box", 10, 1.4 + 0.02j, other=(8, 6)),
+            S.make_job(rng, "ellipsoid", 10, 1.6 + 0.05j, other=(8, 6)),
+            S.make_job(rng, "sphere", 12, 1.5 + 0.1j),
+            S.make_job(rng, "box", 9, 1.3 + 0.01j, other=(7, 5))]
+    alone = D.Server(slots=1, solver=solver, mixed=mixed, eps=1e-9)
+    ref = [alone.wait(S.submit(alone, j)) for j in jobs]
+    alone.close()
+    assert all(r[2]["status"] == 0 and r[2]["packed_rhs"] == 2 for r in ref)
+    srv = D.Server(slots=1, solver=solver, mixed=mixed, eps=1e-9, pack_grids=True)
+    ids = [S.submit(srv, j) for j in jobs]
+    out = [srv.wait(i) for i in ids]
+    srv.close()
+    print([o[2]["packed_rhs"] for o in out], flush=True)
+    assert all(o[2]["status"] == 0 for o in out)
+    assert max(o[2]["packed_rhs"] for o in out) >= 6  # at least three particles in one solve
+    for (p_ref, i_ref, _), (p, infos, _) in zip(ref, out):
+        assert cases.rel_err(p, p_ref) <= 1e-7
+        assert all(i["true_rel_residual"] <= 1.01e-9 for i in infos)
+        # padding entries stay zero
+        assert p.shape == p_ref.shape
