@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -186,6 +188,23 @@ def large_bench(args, D, P, np):
     out["device_bytes_total"] = op64.device_bytes + out["fp32"]["device_bytes"]
     print(json.dumps(out))
     return 0
+
+
+def _plain(o, path="line"):
+    """The output line with numpy scalars and arrays as plain JSON values
+    (complex ones as [re, im], reported on stderr: none is expected)."""
+    if isinstance(o, dict):
+        return {k: _plain(v, "%s.%s" % (path, k)) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_plain(v, "%s[%d]" % (path, i)) for i, v in enumerate(o)]
+    if isinstance(o, np.ndarray):
+        return _plain(o.tolist(), path)
+    if isinstance(o, (complex, np.complexfloating)):
+        print("bench: complex value at %s" % path, file=sys.stderr)
+        return [float(o.real), float(o.imag)]
+    if isinstance(o, np.generic):
+        return o.item()
+    return o
 
 
 def serving_bench(args, D):
@@ -434,7 +453,7 @@ def main():
             sb = D.GpuBicgStab(op, max_iter=20000, mixed=True)
             sb.get_pol_vecs(einc_b, args.eps)  # warm-up
             t0 = time.perf_counter()
-            pb, infos_b = sb.get_pol_vecs(einc_b, args.eps)
+            _, infos_b = sb.get_pol_vecs(einc_b, args.eps)
             el = time.perf_counter() - t0
             sb.close()
             base = modes_out.get("fp64:qmr", {}).get("solves_per_s")
@@ -501,7 +520,7 @@ def main():
         "orientation_batch": orient,
         "serving": serving,
     }
-    print(json.dumps(line))
+    print(json.dumps(_plain(line)))
     return 0
 
 

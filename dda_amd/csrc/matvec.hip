@@ -331,15 +331,54 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
 // (l, t) = (tid mod T, tid / T) owns column c0 + l and the plan's points t + ...
 // ------------------------------------------------------------------------------------
 
+// Buffer resources (V#) for the Z pass: raw (stride 0) buffers whose range
+// check turns loads beyond `bytes` into zeros and drops such stores, so that
+// the zero padding and the cropping need neither branches nor 64-bit address
+// arithmetic per access (the offset is a 32-bit byte offset from a uniform
+// base). Word 3 selects 32-bit data format, as gfx9-family buffers require.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t zrsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 zload(__amdgpu_buffer_rsrc_t r, uint32_t off, float2*) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 zload(__amdgpu_buffer_rsrc_t r, uint32_t off, double2*) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+typedef int zi2 __attribute__((ext_vector_type(2)));
+typedef int zi4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void zstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(zi2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void zstore(__amdgpu_buffer_rsrc_t r, uint32_t off, double2 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(zi4, v), r, off, 0, 0);
+}
+
+// Waves per SIMD the Z pass is compiled for, a register budget: FP32
+// 16-point plans at 3 (168 VGPRs: the tensor loads of several points stay in
+// flight), other FP32 plans at 4, FP64 left to the compiler.
+template <typename R, int PPT> struct P3Waves {
+  static constexpr int value = sizeof(R) == 4 ? (PPT >= 16 ? 3 : 4) : 1;
+};
+
 // Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
+// Thread (l, t) holds, of every component, the points z = t + K_e of the plan's
+// first stage (K_e a compile-time constant per register e), so one lane offset
+// t * Gy * Gx + l addresses all of them from per-(component, e) uniform bases;
+// each such buffer's range ends at z = Bz (points at and beyond it read as zero
+// and are not stored) and spans at most N / PPT planes (it stays below 4 GiB up
+// to 1024^3). The tensor is read through one buffer per component with a lane
+// offset per point.
 template <typename R, int N, int T, bool W>
-__global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_pass3s(typename CxT<R>::T* a2,
+__global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT))
+__attribute__((amdgpu_waves_per_eu(P3Waves<R, StaticPlan<N, W>::PPT>::value))) void k_pass3s(typename CxT<R>::T* a2,
                                                                               const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
                                                 int nper, int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
   typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
+  constexpr int TPC = N / PPT;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Inv;
   const int ntx = (d.Gx + T - 1) / T;
@@ -369,49 +408,60 @@ __global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_pass3s(type
   const int kx = tile_x * T + l;
   const bool colok = kx < d.Gx;
   const long long cstride = (long long)d.Bz * d.Gy * d.Gx;
-  const long long zstride = (long long)d.Gy * d.Gx;
-  C* base = a2 + (long long)ky * d.Gx + kx;
+  const uint32_t zstride = (uint32_t)d.Gy * d.Gx;
+  C* base = a2 + (long long)ky * d.Gx + (long long)tile_x * T;
+  // lanes of columns beyond Gx address past every range
+  const uint32_t voff = colok ? ((uint32_t)t * zstride + l) * (uint32_t)sizeof(C) : 0xfffffff0u;
+  auto range = [&](const C* b, int c, int e) {
+    const int z0 = first_point<N, PPT, F>(0, e);
+    const int nz = d.Bz - z0 < TPC ? d.Bz - z0 : TPC;
+    return zrsrc(b + c * cstride + (long long)z0 * zstride, nz > 0 ? (uint32_t)nz * zstride * sizeof(C) : 0u);
+  };
   C reg[3][PPT];
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
-    for (int e = 0; e < PPT; e++) {
-      const int z = first_point<N, PPT, F>(t, e);
-      reg[c][e] = (!upper_half<F>(e) && colok && z < d.Bz) ? base[c * cstride + z * zstride] : czero<C>();
-    }
+    for (int e = 0; e < PPT; e++)
+      reg[c][e] = upper_half<F>(e) ? czero<C>() : zload(range(base, c, e), voff, (C*)nullptr);
   // Bz <= N/2 (the grid holds at least twice the box): the forward transforms
   // skip the zero upper half and the inverse ones never form it
 #pragma unroll
   for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F, true>::run(reg[c], t, tile, lp, l, tw);
-  const long long oct = (long long)(hx + 1) * (hy + 1) * (hz + 1);
+  const uint32_t plane = (uint32_t)(hy + 1) * (hx + 1);
+  const uint32_t oct = plane * (uint32_t)(hz + 1);
   const int fy = ky <= hy ? ky : d.Gy - ky;
   const R sy = ky <= hy ? (R)1 : (R)-1;
   const int fx = kx <= hx ? kx : d.Gx - kx;
   const R sx = kx <= hx ? (R)1 : (R)-1;
-  if (colok) {
-    const C* Dxy = dhat + (long long)fy * (hx + 1) + fx;
+  const uint32_t fyx = colok ? (uint32_t)fy * (hx + 1) + fx : 0u;
+  const R sxy = sx * sy;
 #pragma unroll
-    for (int e = 0; e < PPT; e++) {
-      const int kz = first_point<N, PPT, Inv>(t, e);
-      const int fz = kz <= hz ? kz : N - kz;
-      const R sz = kz <= hz ? (R)1 : (R)-1;
-      const C* D = Dxy + (long long)fz * (hy + 1) * (hx + 1);
-      const C d0 = D[0], d3 = D[3 * oct], d5 = D[5 * oct];
-      const C d1 = cscale(D[oct], sx * sy), d2 = cscale(D[2 * oct], sx * sz), d4 = cscale(D[4 * oct], sy * sz);
-      const C x0 = reg[0][e], x1 = reg[1][e], x2 = reg[2][e];
-      reg[0][e] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
-      reg[1][e] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
-      reg[2][e] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
-    }
+  for (int e = 0; e < PPT; e++) {
+    const int kz = first_point<N, PPT, Inv>(t, e);
+    const int fz = kz <= hz ? kz : N - kz;
+    const R sz = kz <= hz ? (R)1 : (R)-1;
+    const uint32_t o = ((uint32_t)fz * plane + fyx) * (uint32_t)sizeof(C);
+    C dc[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) dc[c] = zload(zrsrc(dhat + (size_t)c * oct, oct * sizeof(C)), o, (C*)nullptr);
+    const C d0 = dc[0], d1 = cscale(dc[1], sxy), d2 = cscale(dc[2], sx * sz), d3 = dc[3], d4 = cscale(dc[4], sy * sz),
+            d5 = dc[5];
+    const C x0 = reg[0][e], x1 = reg[1][e], x2 = reg[2][e];
+    reg[0][e] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
+    reg[1][e] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
+    reg[2][e] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
   }
+  // the store resources are formed anew (an opaque copy of the base keeps the
+  // compiler from holding the load ones in scalar registers across the
+  // transforms)
+  C* sbase = base;
+  asm volatile("" : "+s"(sbase));
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     RunStages<N, PPT, true, 1, C, Inv>::run(reg[c], t, tile, lp, l, tw);
 #pragma unroll
-    for (int e = 0; e < PPT; e++) {
-      const int z = first_point<N, PPT, F>(t, e);
-      if (!upper_half<F>(e) && colok && z < d.Bz) base[c * cstride + z * zstride] = reg[c][e];
-    }
+    for (int e = 0; e < PPT; e++)
+      if (!upper_half<F>(e)) zstore(range(sbase, c, e), voff, reg[c][e]);
   }
 }
 
