@@ -306,22 +306,33 @@ def main():
         mode, kind, batch = parts[0], parts[1], len(parts) > 2 and parts[2] == "batch"
         solver = D.SOLVER_CLASSES[kind](op, max_iter=20000, mixed=(mode == "mixed"))
 
+        # incident fields resident in HBM before the timed region; the
+        # polarisations stay there until it ends (the boundary's device path)
+        nr, vs = len(rhs), op.vec_size
+        einc_dev = D.DeviceVector(ctx, nr * vs, 64)
+        einc_dev.upload(np.concatenate([r[3] for r in rhs]))
+        pol_dev = D.DeviceVector(ctx, nr * vs, 64)
+        one = [(D.DeviceVector(ctx, vs, 64), D.DeviceVector(ctx, vs, 64)) for _ in rhs]
+        for (e_d, _), r in zip(one, rhs):
+            e_d.upload(r[3])
+
         def step():
-            out = []
             if batch:
                 op.set_couple_constants_batch(np.array([r[2] for r in rhs]))
-                ps, infos = solver.get_pol_vecs(np.array([r[3] for r in rhs]), args.eps)
-                bad = [i for i in infos if i["status"] != D.OK]
-                if bad:
-                    raise RuntimeError("batched solve failed: %s" % bad)
-                for (pol, cc, cs, einc), p, inf in zip(rhs, ps, infos):
-                    out.append((p, inf, cc, einc, pol))
-                return out
-            for pol, cc, cs, einc in rhs:
-                solver.set_couple_constants(cs)
-                p = solver.get_pol_vec(einc, args.eps)
-                out.append((p, dict(solver.info), cc, einc, pol))
-            return out
+                infos = [dict(i) for i in solver.solve_device(einc_dev, pol_dev, nr, args.eps)]
+            else:
+                infos = []
+                for (pol, cc, cs, einc), (e_d, p_d) in zip(rhs, one):
+                    solver.set_couple_constants(cs)
+                    infos.append(dict(solver.solve_device(e_d, p_d, 1, args.eps)[0]))
+            bad = [i for i in infos if i["status"] != D.OK]
+            if bad:
+                raise RuntimeError("solve failed: %s" % bad)
+            return infos
+
+        def results(infos):
+            ps = pol_dev.download().reshape(nr, vs) if batch else [p_d.download() for _, p_d in one]
+            return [(p, inf, cc, einc, pol) for (pol, cc, cs, einc), p, inf in zip(rhs, ps, infos)]
 
         for _ in range(args.warmup):
             step()
@@ -329,16 +340,30 @@ def main():
         op.profile(True)
         op.profile_read()
         t0 = time.perf_counter()
-        res = []
+        infos = []
         for _ in range(args.steps):
-            res = step()
+            infos = step()
         ctx.synchronize()
         el = time.perf_counter() - t0
         prof = solver.profile_read()  # the operator the iteration runs on
         op.profile_read()
         op.profile(False)
+        res = results(infos)
+        # one more step through the host-array path (PCIe copies of the fields
+        # and polarisations included), reported beside the device-resident rate
+        t0 = time.perf_counter()
+        if batch:
+            op.set_couple_constants_batch(np.array([r[2] for r in rhs]))
+            solver.get_pol_vecs(np.array([r[3] for r in rhs]), args.eps)
+        else:
+            for pol, cc, cs, einc in rhs:
+                solver.set_couple_constants(cs)
+                solver.get_pol_vec(einc, args.eps)
+        el_host = time.perf_counter() - t0
         solver.close()
-        return res, el, prof
+        for v in [einc_dev, pol_dev] + [v for pair in one for v in pair]:
+            v.close()
+        return res, el, prof, el_host
 
     runs = {}
     for mode in modes:
@@ -346,7 +371,7 @@ def main():
     final_run = modes[-1]
     final_mode, final_solver = final_run.split(":")[:2]
     final_batch = final_run.endswith(":batch")
-    results, elapsed, _ = runs[final_run]
+    results, elapsed, _, elapsed_host = runs[final_run]
     ms, cnt, skipped = runs[final_run][2]
 
     nsolves = 2 * args.steps
@@ -374,7 +399,7 @@ def main():
 
     obs = observe(results)
     modes_out = {}
-    for mode, (res, el, prof) in runs.items():
+    for mode, (res, el, prof, el_host) in runs.items():
         o = observe(res)
         pms, pcnt, _ = prof
         cm = 8 if mode.startswith("mixed") else 16
@@ -382,6 +407,7 @@ def main():
         pbm = pass_bytes(nv, S, box, grid, rows, cm, km)
         pav = [pms[i] / pcnt[i] if pcnt[i] else float("nan") for i in range(5)]
         modes_out[mode] = {"solves_per_s": nsolves / el, "ms_per_step": 1e3 * el / args.steps,
+                           "solves_per_s_host_arrays": 2 / el_host,
                            "matvec_ms": float(np.nansum(pav)),
                            "rhs_per_matvec": km,
                            "matvec_gbs_vs_bmv": b_mv(nv, box, grid, cm, km) / (float(np.nansum(pav)) * 1e-3) / 1e9,
@@ -451,11 +477,18 @@ def main():
             k = len(ccs_b)
             op.set_couple_constants_batch(ccs_b)
             sb = D.GpuBicgStab(op, max_iter=20000, mixed=True)
-            sb.get_pol_vecs(einc_b, args.eps)  # warm-up
+            e_d = D.DeviceVector(ctx, k * op.vec_size, 64)
+            e_d.upload(einc_b.reshape(-1))
+            p_d = D.DeviceVector(ctx, k * op.vec_size, 64)
+            sb.solve_device(e_d, p_d, k, args.eps)  # warm-up
+            ctx.synchronize()
             t0 = time.perf_counter()
-            _, infos_b = sb.get_pol_vecs(einc_b, args.eps)
+            infos_b = sb.solve_device(e_d, p_d, k, args.eps)
+            ctx.synchronize()
             el = time.perf_counter() - t0
             sb.close()
+            e_d.close()
+            p_d.close()
             base = modes_out.get("fp64:qmr", {}).get("solves_per_s")
             orient = {"workload": "the same sphere, %d seeded directions x 2 polarisations = %d right-hand sides in "
                                   "one block, mixed BiCGStab to %g" % (args.orient_dirs, k, args.eps),
