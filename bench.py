@@ -97,6 +97,7 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np, nrhs=1):
     op32 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=32, grid=params.grid)
     for name, op in (("fp32", op32), ("fp64", op64)):
         op.set_couple_constants_batch(np.array([rhs[r % len(rhs)][2] for r in range(nrhs)]))
+        out[name + "_tuning"] = op.tune(nrhs)  # the kernel variants the solves run
         prec = 32 if name == "fp32" else 64
         x = D.DeviceVector(ctx, nrhs * params.vec_size, prec)
         y = D.DeviceVector(ctx, nrhs * params.vec_size, prec)
@@ -252,6 +253,8 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-step", action="store_true",
+                    help="also time one step through the host-array path (PCIe copies included)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default kernel variants)")
     ap.add_argument("--profile-rhs", type=int, default=1, help="with --matvec-profile: right-hand sides per launch")
     ap.add_argument("--large", action="store_true",
@@ -349,17 +352,19 @@ def main():
         op.profile_read()
         op.profile(False)
         res = results(infos)
-        # one more step through the host-array path (PCIe copies of the fields
-        # and polarisations included), reported beside the device-resident rate
-        t0 = time.perf_counter()
-        if batch:
-            op.set_couple_constants_batch(np.array([r[2] for r in rhs]))
-            solver.get_pol_vecs(np.array([r[3] for r in rhs]), args.eps)
-        else:
-            for pol, cc, cs, einc in rhs:
-                solver.set_couple_constants(cs)
-                solver.get_pol_vec(einc, args.eps)
-        el_host = time.perf_counter() - t0
+        # with --host-step, one more step through the host-array path (PCIe
+        # copies of the fields and polarisations included)
+        el_host = None
+        if args.host_step:
+            t0 = time.perf_counter()
+            if batch:
+                op.set_couple_constants_batch(np.array([r[2] for r in rhs]))
+                solver.get_pol_vecs(np.array([r[3] for r in rhs]), args.eps)
+            else:
+                for pol, cc, cs, einc in rhs:
+                    solver.set_couple_constants(cs)
+                    solver.get_pol_vec(einc, args.eps)
+            el_host = time.perf_counter() - t0
         solver.close()
         for v in [einc_dev, pol_dev] + [v for pair in one for v in pair]:
             v.close()
@@ -407,7 +412,7 @@ def main():
         pbm = pass_bytes(nv, S, box, grid, rows, cm, km)
         pav = [pms[i] / pcnt[i] if pcnt[i] else float("nan") for i in range(5)]
         modes_out[mode] = {"solves_per_s": nsolves / el, "ms_per_step": 1e3 * el / args.steps,
-                           "solves_per_s_host_arrays": 2 / el_host,
+                           "solves_per_s_host_arrays": 2 / el_host if el_host else None,
                            "matvec_ms": float(np.nansum(pav)),
                            "rhs_per_matvec": km,
                            "matvec_gbs_vs_bmv": b_mv(nv, box, grid, cm, km) / (float(np.nansum(pav)) * 1e-3) / 1e9,
