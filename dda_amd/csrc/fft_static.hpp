@@ -257,13 +257,18 @@ template <int R, int... Rs> struct Reverse<RadixList<R, Rs...>> {
 // twiddle W_{NS R}^k comes from the table of N roots of unity, its powers by
 // repeated multiplication. HALF (first stage only): inputs m >= R/2 are zero
 // and not read.
-template <int N, int PPT, int R, int NS, bool INV, bool HALF, typename C>
+// PRE: tw holds the R - 1 twiddle powers of the thread's one butterfly of the
+// twiddled stage (formed once by the caller and shared by all its transforms).
+template <int N, int PPT, int R, int NS, bool INV, bool HALF, typename C, bool PRE = false>
 __device__ inline void stage_compute(C* v, int t, const C* __restrict__ tw) {
   constexpr int TPC = N / PPT;
 #pragma unroll
   for (int i = 0; i < PPT / R; i++) {
     C* a = v + i * R;
-    if (NS > 1) {
+    if (NS > 1 && PRE) {
+#pragma unroll
+      for (int m = 1; m < R; m++) a[m] = INV ? cmulc(a[m], tw[m - 1]) : cmul(a[m], tw[m - 1]);
+    } else if (NS > 1) {
       const int j = t + i * TPC;
       const int k = j % NS;  // NS is a compile-time constant
       const C w1 = tw[k * (N / (NS * R))];
@@ -313,26 +318,47 @@ __device__ inline void stage_load(C* v, int t, const C* tile, int lp, int l) {
 // any exchange happened, so the caller may reuse the tile. HALF: the upper half
 // of every first-stage butterfly's inputs is zero (a zero-padded transform of
 // at most N/2 points) and is neither read nor computed with.
-template <int N, int PPT, bool INV, int NS, typename C, typename L, bool HALF = false> struct RunStages;
+// PRE (two-stage plans with one butterfly per thread per stage, see
+// TwoStage): tw is the thread's R2 - 1 second-stage twiddle powers.
+template <int N, int PPT, bool INV, int NS, typename C, typename L, bool HALF = false, bool PRE = false>
+struct RunStages;
 
-template <int N, int PPT, bool INV, int NS, typename C, int R, bool HALF>
-struct RunStages<N, PPT, INV, NS, C, RadixList<R>, HALF> {
+template <int N, int PPT, bool INV, int NS, typename C, int R, bool HALF, bool PRE>
+struct RunStages<N, PPT, INV, NS, C, RadixList<R>, HALF, PRE> {
   static __device__ inline void run(C* v, int t, C* tile, int lp, int l, const C* tw) {
-    stage_compute<N, PPT, R, NS, INV, HALF>(v, t, tw);
+    stage_compute<N, PPT, R, NS, INV, HALF, C, PRE>(v, t, tw);
   }
 };
 
-template <int N, int PPT, bool INV, int NS, typename C, int R, int R2, int... Rs, bool HALF>
-struct RunStages<N, PPT, INV, NS, C, RadixList<R, R2, Rs...>, HALF> {
+template <int N, int PPT, bool INV, int NS, typename C, int R, int R2, int... Rs, bool HALF, bool PRE>
+struct RunStages<N, PPT, INV, NS, C, RadixList<R, R2, Rs...>, HALF, PRE> {
   static __device__ inline void run(C* v, int t, C* tile, int lp, int l, const C* tw) {
-    stage_compute<N, PPT, R, NS, INV, HALF>(v, t, tw);
+    stage_compute<N, PPT, R, NS, INV, HALF, C, PRE>(v, t, tw);
     stage_store<N, PPT, R, NS>(v, t, tile, lp, l);
     __syncthreads();
     stage_load<N, PPT, R2>(v, t, tile, lp, l);
     __syncthreads();
-    RunStages<N, PPT, INV, NS * R, C, RadixList<R2, Rs...>, false>::run(v, t, tile, lp, l, tw);
+    RunStages<N, PPT, INV, NS * R, C, RadixList<R2, Rs...>, false, PRE>::run(v, t, tile, lp, l, tw);
   }
 };
+
+// Plans of two stages with PPT = R0 = R1 (a thread holds one butterfly of
+// each stage; its second-stage twiddles are W_N^(t m), m = 1 .. R1 - 1, the
+// same for the forward and the conjugated inverse transform).
+template <int N, int PPT, typename L> struct TwoStage { static constexpr bool value = false; };
+template <int N, int PPT, int R0, int R1> struct TwoStage<N, PPT, RadixList<R0, R1>> {
+  static constexpr bool value = PPT == R0 && PPT == R1 && R0 * R1 == N;
+};
+template <int R, typename C>
+__device__ inline void twiddle_powers(C* wp, int t, const C* __restrict__ tw) {
+  const C w1 = tw[t % R];
+  C w = w1;
+#pragma unroll
+  for (int m = 0; m < R - 1; m++) {
+    wp[m] = w;
+    if (m + 2 < R) w = cmul(w, w1);
+  }
+}
 
 template <typename L> struct FirstRadix;
 template <int R, int... Rs> struct FirstRadix<RadixList<R, Rs...>> { static constexpr int value = R; };

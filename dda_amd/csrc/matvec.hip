@@ -424,9 +424,14 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, StaticPlan<N, W>::PPT>::value))) v
     for (int e = 0; e < PPT; e++)
       reg[c][e] = upper_half<F>(e) ? czero<C>() : zload(range(base, c, e), voff, (C*)nullptr);
   // Bz <= N/2 (the grid holds at least twice the box): the forward transforms
-  // skip the zero upper half and the inverse ones never form it
+  // skip the zero upper half and the inverse ones never form it; two-stage
+  // plans form the second-stage twiddle powers once for all six transforms
+  constexpr bool PRE = TwoStage<N, PPT, F>::value && sizeof(R) == 4;  // FP64: no registers to spare
+  C wp[PRE ? PPT - 1 : 1];
+  if constexpr (PRE) twiddle_powers<PPT>(wp, t, tw);
+  const C* twp = PRE ? wp : tw;
 #pragma unroll
-  for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F, true>::run(reg[c], t, tile, lp, l, tw);
+  for (int c = 0; c < 3; c++) RunStages<N, PPT, false, 1, C, F, true, PRE>::run(reg[c], t, tile, lp, l, twp);
   const uint32_t plane = (uint32_t)(hy + 1) * (hx + 1);
   const uint32_t oct = plane * (uint32_t)(hz + 1);
   const int fy = ky <= hy ? ky : d.Gy - ky;
@@ -458,7 +463,7 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, StaticPlan<N, W>::PPT>::value))) v
   asm volatile("" : "+s"(sbase));
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    RunStages<N, PPT, true, 1, C, Inv>::run(reg[c], t, tile, lp, l, tw);
+    RunStages<N, PPT, true, 1, C, Inv, false, PRE>::run(reg[c], t, tile, lp, l, twp);
 #pragma unroll
     for (int e = 0; e < PPT; e++)
       if (!upper_half<F>(e)) zstore(range(sbase, c, e), voff, reg[c][e]);
