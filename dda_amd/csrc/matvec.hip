@@ -347,11 +347,34 @@ __device__ __forceinline__ double2 zload(__amdgpu_buffer_rsrc_t r, uint32_t off,
 }
 typedef int zi2 __attribute__((ext_vector_type(2)));
 typedef int zi4 __attribute__((ext_vector_type(4)));
+constexpr int kStoreNT = 2;  // buffer cache policy: nt
+// The Z pass's stores and those of the X-forward and Y-inverse passes are
+// non-temporal (nt): their outputs are read by the next pass only after this
+// kernel's end, and streaming them leaves little dirty data in the L2s to be
+// written back when the kernel ends (the gap before the next kernel).
+// Measured in a solve at 256^3: 591 -> 560 us per 2-RHS matvec; the Y-forward
+// pass keeps ordinary stores (non-temporal ones made it 5% slower).
 __device__ __forceinline__ void zstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(zi2, v), r, off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(zi2, v), r, off, 0, kStoreNT);
 }
 __device__ __forceinline__ void zstore(__amdgpu_buffer_rsrc_t r, uint32_t off, double2 v) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(zi4, v), r, off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(zi4, v), r, off, 0, kStoreNT);
+}
+template <bool NT> __device__ __forceinline__ void cstore(float2* p, float2 v) {
+  if constexpr (NT) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(__builtin_bit_cast(f2, v), reinterpret_cast<f2*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <bool NT> __device__ __forceinline__ void cstore(double2* p, double2 v) {
+  if constexpr (NT) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(__builtin_bit_cast(d2, v), reinterpret_cast<d2*>(p));
+  } else {
+    *p = v;
+  }
 }
 
 // Waves per SIMD the Z pass is compiled for, a register budget: FP32
@@ -524,7 +547,7 @@ __global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __rest
   if (g < nlines) {
     C* dst = a1 + (size_t)g * N;
 #pragma unroll
-    for (int e = 0; e < PPT; e++) dst[first_point<N, PPT, Out>(t, e)] = v[e];
+    for (int e = 0; e < PPT; e++) cstore<true>(dst + first_point<N, PPT, Out>(t, e), v[e]);
   }
 }
 
@@ -624,7 +647,8 @@ __global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT)) void k_colfft_s(co
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const int p = first_point<N, PPT, Out>(t, e);
-    if (!(HOUT && upper_half<Out>(e)) && colok && p < cp.out_len) dst[p * cp.pstride_out] = v[e];
+    if (!(HOUT && upper_half<Out>(e)) && colok && p < cp.out_len)
+      cstore<INV>(dst + p * cp.pstride_out, v[e]);
   }
 }
 
