@@ -537,6 +537,10 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
       break;
     }
     case S_END_ITER: {  // IterativeSolverBase.cpp:62-70 then the loop head
+      if (kind == DDA_SOLVER_BICGSTAB && !st->skip) {  // |r|^2 of the x/r update
+        st->ip1 = q0;
+        st->ro_old = st->ro_new;
+      }
       const double ip1 = (kind == DDA_SOLVER_BICGSTAB) ? st->ip1 : q0;
       st->count++;
       if (ip1 <= st->inprodR) {
@@ -623,19 +627,6 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
   }
 }
 
-// BiCGStab: x = p alpha + x, only on the early exit of an iteration.
-template <typename R>
-__global__ void k_bs_early(typename CxT<R>::T* x, const typename CxT<R>::T* __restrict__ p, long long n,
-                           const SolverState* st) {
-  typedef typename CxT<R>::T C;
-  DDA_RHS_PROLOGUE(st)
-  if (s_->done || !s_->skip) return;
-  x += off_;
-  p += off_;
-  const C a = coef<C>(&s_->alpha);
-  DDA_GRID_LOOP(i) x[i] = cadd(cmul(p[i], a), x[i]);
-}
-
 // BiCGStab: p = p beta + v (-beta omega) + r (nr > 0) or p = r.
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_bs_p(typename CxT<R>::T* p, const typename CxT<R>::T* __restrict__ v,
@@ -675,7 +666,9 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_om(const typename CxT<R>::T*
   write_partials(part, 3, n2, dr, di);
 }
 
-// BiCGStab: x = p alpha + s omega + x; r = Av (-omega) + s; partial |r|^2.
+// BiCGStab: x = p alpha + s omega + x; r = Av (-omega) + s; partial |r|^2. On
+// the early exit of an iteration (|s| small, BicgStab.cpp:109-110) only
+// x = p alpha + x.
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, typename CxT<R>::T* r,
                                                         const typename CxT<R>::T* __restrict__ p,
@@ -684,13 +677,17 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, t
                                                         double* part, const SolverState* st) {
   typedef typename CxT<R>::T C;
   DDA_RHS_PROLOGUE(st)
-  if (s_->skip) return;
+  if (s_->done) return;
   x += off_;
   r += off_;
   p += off_;
   s += off_;
   av += off_;
   const C a = coef<C>(&s_->alpha), om = coef<C>(&s_->omega), mom = coef<C>(&s_->mOmega);
+  if (s_->skip) {
+    DDA_GRID_LOOP(i) x[i] = cadd(cmul(p[i], a), x[i]);
+    return;
+  }
   double sum = 0;
   DDA_GRID_LOOP(i) {
     x[i] = cadd(cadd(cmul(p[i], a), cmul(s[i], om)), x[i]);
@@ -699,17 +696,6 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, t
     sum += (double)rn.x * rn.x + (double)rn.y * rn.y;
   }
   write_partials(part, 1, sum, 0, 0);
-}
-
-// BiCGStab end of iteration: ip1 from |r|^2 unless the early exit was taken.
-__global__ __launch_bounds__(kRedThreads) void k_bs_fin(SolverState* st_all, const double* part_all, int nb) {
-  SolverState* st = st_all + blockIdx.x;
-  if (st->done) return;
-  const double q0 = sum_partials(part_all + (long long)blockIdx.x * 4 * nb, nb, 0);
-  if (threadIdx.x == 0 && !st->skip) {
-    st->ip1 = q0;
-    st->ro_old = st->ro_new;
-  }
 }
 
 // CGNR / BiCG: p = first ? a : p beta + a.
@@ -985,12 +971,10 @@ struct Core {
           scalar(S_BS_ALPHA);
           lincomb(sv, vv, DDA_OFF(mAlpha), nullptr, -1, r, true, DONE);  // s = v (-alpha) + r
           scalar(S_BS_S);
-          hipLaunchKernelGGL(k_bs_early<R>, g(), b(), 0, st, x, p, n, dst);
           DDA_TRY(matvec(sv, av, 0, SKIP));
           hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, av, sv, n, part, dst);
           scalar(S_BS_OMEGA);
           hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, n, part, dst);
-          hipLaunchKernelGGL(k_bs_fin, dim3(nrhs), b(), 0, st, dst, part, nb);
           scalar(S_END_ITER);
           break;
         }
