@@ -373,6 +373,7 @@ enum Step {
   S_BS_S = 22,
   S_BS_OMEGA = 23,
   S_BS_END = 24,
+  S_BS_RO1 = 25,     // S_BS_RO with r.conj(rtilda) from the previous x/r update (partials 1, 2)
   S_CG_RO = 30,
   S_CG_ALPHA = 31,
   S_BI_RO = 40,
@@ -441,6 +442,7 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
       break;
     case S_QMR_GIVENS:
     case S_BS_OMEGA:
+    case S_BS_RO1:
       q0 = sum_partials(part, nb, 0);
       q1 = sum_partials(part, nb, 1);
       q2 = sum_partials(part, nb, 2);
@@ -554,8 +556,9 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
       break;
     }
     // ---- BiCGStab, BicgStab.cpp:66-112 ----
-    case S_BS_RO: {
-      st->ro_new = make_double2(q0, q1);
+    case S_BS_RO:
+    case S_BS_RO1: {
+      st->ro_new = step == S_BS_RO1 ? make_double2(q1, q2) : make_double2(q0, q1);
       const double dtmp = dabs(st->ro_new) / st->inprodR;
       if (!(dtmp >= 1e-16)) {
         finish(st, DDA_BREAKDOWN);
@@ -666,14 +669,16 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_om(const typename CxT<R>::T*
   write_partials(part, 3, n2, dr, di);
 }
 
-// BiCGStab: x = p alpha + s omega + x; r = Av (-omega) + s; partial |r|^2. On
+// BiCGStab: x = p alpha + s omega + x; r = Av (-omega) + s; partials |r|^2
+// and r.conj(rtilda) (the next iteration's rho). On
 // the early exit of an iteration (|s| small, BicgStab.cpp:109-110) only
 // x = p alpha + x.
 template <typename R>
 __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, typename CxT<R>::T* r,
                                                         const typename CxT<R>::T* __restrict__ p,
                                                         const typename CxT<R>::T* __restrict__ s,
-                                                        const typename CxT<R>::T* __restrict__ av, long long n,
+                                                        const typename CxT<R>::T* __restrict__ av,
+                                                        const typename CxT<R>::T* __restrict__ rt, long long n,
                                                         double* part, const SolverState* st) {
   typedef typename CxT<R>::T C;
   DDA_RHS_PROLOGUE(st)
@@ -688,14 +693,20 @@ __global__ __launch_bounds__(kRedThreads) void k_bs_upd(typename CxT<R>::T* x, t
     DDA_GRID_LOOP(i) x[i] = cadd(cmul(p[i], a), x[i]);
     return;
   }
-  double sum = 0;
+  rt += off_;
+  double sum = 0, d0 = 0, d1 = 0;
   DDA_GRID_LOOP(i) {
     x[i] = cadd(cadd(cmul(p[i], a), cmul(s[i], om)), x[i]);
     const C rn = cadd(cmul(av[i], mom), s[i]);
     r[i] = rn;
-    sum += (double)rn.x * rn.x + (double)rn.y * rn.y;
+    const double ar = rn.x, ai = rn.y;
+    sum += ar * ar + ai * ai;
+    // the next iteration's r.conj(rtilda), summed as k_reduce<RED_DOTC> does
+    const double br = rt[i].x, bi = -(double)rt[i].y;
+    d0 += ar * br - ai * bi;
+    d1 += ar * bi + ai * br;
   }
-  write_partials(part, 1, sum, 0, 0);
+  write_partials(part, 3, sum, d0, d1);
 }
 
 // CGNR / BiCG: p = first ? a : p beta + a.
@@ -963,8 +974,12 @@ struct Core {
           C* vv = v[5];
           C* sv = v[6];
           C* rt = v[7];
-          reduce(RED_DOTC, r, rt, DONE);
-          scalar(S_BS_RO);
+          if (nr == 0) {
+            reduce(RED_DOTC, r, rt, DONE);
+            scalar(S_BS_RO);
+          } else {
+            scalar(S_BS_RO1);  // rho from the previous x/r update's partials
+          }
           hipLaunchKernelGGL(k_bs_p<R>, g(), b(), 0, st, p, vv, r, nr == 0 ? 1 : 0, dst, n);
           DDA_TRY(matvec(p, vv, 0, DONE));
           reduce(RED_DOTC, vv, rt, DONE);
@@ -974,7 +989,7 @@ struct Core {
           DDA_TRY(matvec(sv, av, 0, SKIP));
           hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, av, sv, n, part, dst);
           scalar(S_BS_OMEGA);
-          hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, n, part, dst);
+          hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, rt, n, part, dst);
           scalar(S_END_ITER);
           break;
         }
