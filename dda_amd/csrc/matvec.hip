@@ -2358,15 +2358,31 @@ extern "C" dda_status dda_op_profile_read(dda_op_t op, double ms[5], uint64_t co
     ms[i] = 0;
     counts[i] = 0;
   }
+  // Launches of a device-side solve after some or all of its right-hand sides
+  // have finished do less work or none (kernels return at entry for finished
+  // ones): launches shorter than 60% of their pass's median are counted in
+  // `skipped`, not in the times, so that ms / counts is the full launch.
   uint64_t sk = 0;
-  for (auto& r : op->prof_recs) {
+  std::vector<float> t(op->prof_recs.size());
+  std::vector<float> by_pass[5];
+  for (size_t i = 0; i < op->prof_recs.size(); i++) {
+    auto& r = op->prof_recs[i];
     DDA_TRY_HIP(hipEventSynchronize(r.e));
-    float t = 0;
-    DDA_TRY_HIP(hipEventElapsedTime(&t, r.b, r.e));
-    if (t < 0.002f) {
+    DDA_TRY_HIP(hipEventElapsedTime(&t[i], r.b, r.e));
+    by_pass[r.pass].push_back(t[i]);
+  }
+  float med[5] = {0, 0, 0, 0, 0};
+  for (int p = 0; p < 5; p++)
+    if (!by_pass[p].empty()) {
+      std::nth_element(by_pass[p].begin(), by_pass[p].begin() + by_pass[p].size() / 2, by_pass[p].end());
+      med[p] = by_pass[p][by_pass[p].size() / 2];
+    }
+  for (size_t i = 0; i < op->prof_recs.size(); i++) {
+    auto& r = op->prof_recs[i];
+    if (t[i] < 0.6f * med[r.pass] || t[i] < 0.002f) {
       sk++;
     } else {
-      ms[r.pass] += t;
+      ms[r.pass] += t[i];
       counts[r.pass]++;
     }
     op->prof_pool.push_back(r.b);
