@@ -140,6 +140,7 @@ def large_bench(args, D, P, np):
                                                  grid=params.grid)
         op.set_couple_constants(cs)
         t_create = time.perf_counter() - t0
+        tuning = op.tune(1) if not args.no_tune else "not tuned"
         x = D.DeviceVector(ctx, params.vec_size, prec)
         y = D.DeviceVector(ctx, params.vec_size, prec)
         x.upload(P.random_dipole_vector(params))
@@ -160,7 +161,7 @@ def large_bench(args, D, P, np):
                      "pass_gbs": [pb[i] / (pav[i] * 1e-3) / 1e9 for i in range(5)],
                      "gbs_vs_bmv": bmv / (mv * 1e-3) / 1e9, "frac_vs_bmv": bmv / (mv * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "gbs_algorithmic": sum(pb) / (mv * 1e-3) / 1e9, "device_bytes": op.device_bytes,
-                     "create_s": t_create}
+                     "create_s": t_create, "tuning": tuning}
         print("large %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
         x.close()
         y.close()
