@@ -513,6 +513,19 @@ def main():
             serving = {"error": str(e)}
 
     info = ctx.info()
+    # what a plain device-to-device copy sustains on this GPU (1 GiB, read +
+    # write counted): the practical ceiling of a streaming pass
+    na = (1 << 30) // 16
+    ca, cb = D.DeviceVector(ctx, na, 64), D.DeviceVector(ctx, na, 64)
+    cb.copy_from(ca)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        cb.copy_from(ca)
+    ctx.synchronize()
+    copy_gbs = 2 * (1 << 30) * 10 / (time.perf_counter() - t0) / 1e9
+    ca.close()
+    cb.close()
     line = {
         "metric": METRIC,
         "value": value,
@@ -536,6 +549,7 @@ def main():
                    "eps": args.eps,
                    "baseline": "fp64:qmr (the reference's default solver, all FP64)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "copy_gbs": copy_gbs,
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.diameter, grid, "fp32" if c == 8 else "fp64", dom, kb),
                      "traffic_source": "profiles/traffic_matvec_sphere128.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
