@@ -509,7 +509,7 @@ def main():
     if args.serve_jobs > 0:
         try:
             serving = serving_bench(args, D)
-        except Exception as e:  # reported, not fatal: the headline is the C2 solve rate
+        except Exception as e:  # reported, not fatal: the headline is the solve rate of the 128-across sphere
             serving = {"error": str(e)}
 
     info = ctx.info()

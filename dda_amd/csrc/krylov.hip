@@ -1098,7 +1098,7 @@ struct Core {
         // half the iterations the recent convergence rate still asks for
         // (the residuals of these methods are erratic, and a longer guess
         // runs past convergence as short no-op launches), at least one graph
-        // pair; measured 1% faster than the full estimate on a 2-RHS C2 solve
+        // pair; measured 1% faster than the full estimate on a 2-RHS solve at 256^3
         int li = kChunk / 2;
         if (prev_r[i] > 0 && hs[i].inprodR > 0 && hs[i].count > prev_count[i] && hs[i].inprodR < prev_r[i] &&
             hs[i].epsB > 0) {

@@ -78,7 +78,7 @@ def field_diff(a, b):
 
 
 def angle_grid(theta_step_deg=2.0, phi_step_deg=6.0):
-    """theta 0..180 and phi 0..(360 - step) on a regular grid (the C1 Jones grid
+    """theta 0..180 and phi 0..(360 - step) on a regular grid (the Jones grid of the 40-across sphere case
     is 2 deg x 6 deg) -> (theta0, dtheta, ntheta, phi0, dphi, nphi) in radians."""
     nt = int(round(180.0 / theta_step_deg)) + 1
     npf = int(round(360.0 / phi_step_deg))

@@ -139,7 +139,7 @@ def test_c3_ellipsoid_128_rhs(ctx):
     op.set_couple_constants_batch(ccs)
     s = D.GpuQmrCs(op, mixed=True)
     pb, info = s.get_pol_vecs(einc, 1e-6)
-    print("C3 iterations", [i["iterations"] for i in info][:8], "seconds", info[0]["seconds"], flush=True)
+    print("ellipsoid batch iterations", [i["iterations"] for i in info][:8], "seconds", info[0]["seconds"], flush=True)
     bad = [(r, i) for r, i in enumerate(info) if i["status"] != D.OK]
     assert not bad, bad[:4]
     assert max(i["true_rel_residual"] for i in info) < 1e-6

@@ -108,7 +108,7 @@ def test_host_dmatrix_input(ctx):
 
 
 def test_matvec_c1_full_size(ctx):
-    """C1: sphere 40 dipoles across (33,552 dipoles), 80^3 grid, FP64 and FP32."""
+    """Sphere 40 dipoles across (33,552 dipoles), 80^3 grid, FP64 and FP32."""
     g, params, cc, cs, prop, pol = cases.build("sphere_c1")
     assert params.nv == 33552 and params.grid == (80, 80, 80)
     orc = cases.oracle_problem(g, params, cs)

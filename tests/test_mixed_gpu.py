@@ -42,7 +42,7 @@ def test_mixed_matches_fp64_oracle(ctx, kind, name):
 
 
 def test_mixed_c1_to_1e8(ctx):
-    """C1 (40 across, 80^3) to 1e-8 in mixed precision: several refinement steps."""
+    """Sphere 40 across, 80^3, to 1e-8 in mixed precision: several refinement steps."""
     g, params, cc, cs, prop, pol = cases.build("sphere_c1")
     einc = P.einc_plane(params, prop, pol)
     op = _op(ctx, g, params, cs)

@@ -257,7 +257,7 @@ def test_ldr_limits():
 
 
 def test_geometry_counts():
-    # SURVEY figures: 40 across -> 33,552 dipoles; 128 across -> 1,099,136
+    # the published workload sizes: 40 across -> 33,552 dipoles; 128 across -> 1,099,136
     pos, box = O.geometry("sphere", 20.0)
     assert len(pos) == 33552 and list(box) == [40, 40, 40]
     pos, box = O.geometry("box", 10, 8, 6)

@@ -49,7 +49,7 @@ def test_solver_matches_oracle(ctx, kind, name):
 
 
 def test_qmr_c1_solve(ctx):
-    """C1: sphere 40 across, QMR-CS to 1e-8; Cext within 1e-6 relative of the oracle."""
+    """Sphere 40 across, QMR-CS to 1e-8; Cext within 1e-6 relative of the oracle."""
     g, params, cc, cs, prop, pol = cases.build("sphere_c1")
     einc = P.einc_plane(params, prop, pol)
     s = D.GpuQmrCs(_op(ctx, g, params, cs), max_iter=2000)
