@@ -280,3 +280,120 @@ def test_oracle_matvec_thread_count_independent():
     y4 = orc.matvec(x, precision=64)
     O.set_threads(1)
     assert np.array_equal(y1, y4)
+
+
+def _csca(P, pvec, nt=24, nph=48):
+    """Integral of |E_far|^2 over the unit sphere: Gauss-Legendre in cos(theta),
+    uniform (spectrally accurate) in phi."""
+    x, w = np.polynomial.legendre.leggauss(nt)
+    tot = 0.0
+    for ct, wt in zip(x, w):
+        st = math.sqrt(max(0.0, 1 - ct * ct))
+        for j in range(nph):
+            ph = 2 * math.pi * j / nph
+            e = P.calc_field(pvec, (st * math.cos(ph), st * math.sin(ph), ct))
+            tot += wt * (2 * math.pi / nph) * float(np.sum(np.abs(e) ** 2))
+    return tot
+
+
+@pytest.mark.parametrize("shape,p,m", [("box", (5, 4, 3), 1.5 + 0.1j), ("sphere", (3.0, 0, 0), 1.5 + 0.1j),
+                                       ("box", (5, 4, 3), 1.7)])
+def test_energy_balance_and_jones_normalisation(shape, p, m):
+    """Energy conservation of the dipole solution pins the observables against
+    each other: Cext (the sum of Im(p.E*), already pinned by the optical
+    theorem) = Cabs + Csca per polarisation, with Csca the integral of |E_far|^2
+    over all directions (a quadrature independent of any of the three), and
+    Cabs = 0 for a real m (LDR's radiative reaction). The Jones matrices of a
+    grid of angles integrate to Csca(pol 1) + Csca(pol 2)."""
+    d, lam = 0.5e-6, 6e-6
+    k = 2 * math.pi / lam
+    pos, box = O.geometry(shape, *p)
+    g = np.array([O.fft_fit(2 * int(b)) for b in box], np.uint32)
+    prop, pol1, pol2 = O.beam_pols([0, 0, 1])
+    csca = []
+    ps = []
+    for pol in (pol1, pol2):
+        cc, cs = O.couple_constants("ldr", m, d, k * d, prop, pol)
+        P = O.Problem(pos, None, box, g, cs, d, k)
+        e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol)
+        st, pvec, _, _ = P.solve("qmr", e, 1e-12, max_iter=3000)
+        assert st == 0
+        cext = P.ext_cross_sum(pvec, e)
+        cabs = P.abs_cross(pvec, cc)
+        cs_ = _csca(P, pvec)
+        assert abs(cext - (cabs + cs_)) <= 1e-7 * cext, (cext, cabs, cs_)
+        if np.imag(m) == 0:
+            assert abs(cabs) <= 1e-9 * cext
+        else:
+            assert cabs > 0.1 * cext
+        csca.append(cs_)
+        ps.append(pvec)
+    # Jones: composite Simpson in theta (sin weight), periodic trapezoid in phi
+    nt, nph = 181, 48
+    dt, dp = math.pi / (nt - 1), 2 * math.pi / nph
+    J = P.jones(ps[0], ps[1], 0, prop, pol1, pol2, nt, 0.0, dt, nph, 0.0, dp).reshape(nt, nph, 4)
+    wt = np.ones(nt)
+    wt[1:-1:2], wt[2:-1:2] = 4, 2
+    wt *= dt / 3 * np.sin(np.arange(nt) * dt)
+    tot = float(np.sum(wt[:, None] * np.sum(np.abs(J) ** 2, axis=2)) * dp)
+    assert abs(tot - (csca[0] + csca[1])) <= 1e-6 * tot, (tot, csca)
+
+
+def test_jones_element_placement_on_a_mirror_symmetric_target():
+    """The lattice sphere is mirror-symmetric about the planes through its centre
+    that contain the propagation axis and a lattice axis, so in those scattering
+    planes (phi = 0 and 90 degrees) the off-diagonal Jones elements (s3 = J[0][1],
+    s4 = J[1][0]) vanish and the diagonal ones do not; in the forward direction
+    the two diagonal elements are equal and give Cext by the optical theorem:
+    Cext = -(4 pi / k) Re J_diag(0) for the polarisation along that element."""
+    d, lam, m = 0.5e-6, 6e-6, 1.5 + 0.1j
+    k = 2 * math.pi / lam
+    pos, box = O.geometry("sphere", 3.0)
+    g = np.array([O.fft_fit(2 * int(b)) for b in box], np.uint32)
+    prop, pol1, pol2 = O.beam_pols([0, 0, 1])
+    ps, cext = [], []
+    for pol in (pol1, pol2):
+        cc, cs = O.couple_constants("ldr", m, d, k * d, prop, pol)
+        P = O.Problem(pos, None, box, g, cs, d, k)
+        e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol)
+        st, pvec, _, _ = P.solve("qmr", e, 1e-12, max_iter=3000)
+        ps.append(pvec)
+        cext.append(P.ext_cross_sum(pvec, e))
+    nt = 19
+    J = P.jones(ps[0], ps[1], 0, prop, pol1, pol2, nt, 0.0, math.pi / (nt - 1), 2, 0.0, math.pi / 2)
+    J = J.reshape(nt, 2, 2, 2)  # [theta][phi 0, 90][row][col]
+    diag = np.abs(J[:, :, 0, 0]) + np.abs(J[:, :, 1, 1])
+    off = np.abs(J[:, :, 0, 1]) + np.abs(J[:, :, 1, 0])
+    assert np.all(off <= 1e-9 * diag.max()) and np.all(diag > 1e-3 * diag.max())
+    f = J[0, 0]
+    assert abs(f[0, 0] - f[1, 1]) <= 1e-9 * abs(f[0, 0])
+    # the sphere scatters both polarisations alike: Cext equal, and from the forward amplitude
+    assert abs(cext[0] - cext[1]) <= 1e-9 * cext[0]
+    assert abs(-4 * math.pi / k * f[0, 0].real - cext[0]) <= 1e-8 * cext[0]
+
+
+def test_beam_pols_and_plane_wave():
+    """Beam.cpp:34-51: (prop, pol1, pol2) is an orthonormal triad for any
+    direction (pol1 = y, pol2 = x for +z, the reference's own choice); Beam.cpp:110-118:
+    the plane wave has unit amplitude along pol at every dipole and advances in
+    phase by k d per grid step along prop, with phase 0 at the box centre."""
+    rng = np.random.default_rng(9)
+    for v in [(0, 0, 1), (0, 0, -1), (1, 0, 0)] + [tuple(rng.normal(size=3)) for _ in range(5)]:
+        prop, p1, p2 = O.beam_pols(v)
+        M = np.array([prop, p1, p2])
+        assert np.allclose(M @ M.T, np.eye(3), atol=1e-14)
+        assert np.allclose(prop, np.asarray(v) / np.linalg.norm(v), atol=1e-14)
+        # pol1 is horizontal (no z part) and pol2 = pol1 x prop (ADDA's frame)
+        assert abs(p1[2]) <= 1e-14 and np.allclose(p2, np.cross(p1, prop), atol=1e-14)
+    prop, p1, p2 = O.beam_pols([0, 0, 1])
+    assert np.allclose(p1, [0, 1, 0]) and np.allclose(p2, [1, 0, 0])
+    _, q1, q2 = O.beam_pols([0, 0, -1])
+    assert np.allclose(q1, [0, 1, 0]) and np.allclose(q2, [-1, 0, 0])
+    pos, box = O.geometry("box", 4, 3, 5)
+    d, k = 0.5e-6, 2 * math.pi / 6e-6
+    vs = (len(pos) + 15) // 16 * 16
+    e = O.einc_plane(pos, box, vs, d, k, prop, p1)
+    E = np.stack([e[c * vs: c * vs + len(pos)] for c in range(3)], axis=1)
+    assert np.allclose(np.abs(E[:, 1]), 1, atol=1e-15) and np.all(E[:, 0] == 0) and np.all(E[:, 2] == 0)
+    z = pos[:, 2].astype(float) - (box[2] - 1) / 2.0
+    assert np.allclose(E[:, 1], np.exp(1j * k * d * z), atol=1e-14)
