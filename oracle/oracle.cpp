@@ -33,6 +33,13 @@
 //   orc_cc             Clausius-Mossotti closed form; LDR -> CM as kd -> 0
 //   orc_mie            optical theorem / known Qext values (textbook)
 //   orc_ext_cross_*    the two Cext formulas agree (optical theorem)
+//   orc_abs_cross,     energy conservation Cext = Cabs + Csca (Csca by quadrature
+//   orc_calc_field     of |E_far|^2); Cabs = 0 for real m
+//   orc_jones          integral of sum |J_ij|^2 = Csca(pol 1) + Csca(pol 2); zero
+//                      off-diagonals on the lattice sphere's mirror planes; forward
+//                      amplitude gives Cext
+//   orc_beam_pols,     orthonormal frame, pol1 horizontal, pol2 = pol1 x prop;
+//   orc_einc_plane     unit amplitude, phase k d per step along prop
 // Exact iteration counts and residual histories of the reference (built with
 // -ffast-math and FFTW_ESTIMATE plans) are parity unpinned to the bit; they are
 // pinned to tolerance only.
