@@ -379,9 +379,11 @@ template <bool NT> __device__ __forceinline__ void cstore(double2* p, double2 v)
 
 // Waves per SIMD the Z pass is compiled for, a register budget: FP32
 // 16-point plans at 3 (168 VGPRs: the tensor loads of several points stay in
-// flight), other FP32 plans at 4, FP64 left to the compiler.
-template <typename R, int PPT> struct P3Waves {
-  static constexpr int value = sizeof(R) == 4 ? (PPT >= 16 ? 3 : 4) : 1;
+// flight), other FP32 plans at 4, FP64 left to the compiler. At 1024 points
+// the 16-point plan's 512-thread work-groups fit one per CU at 3 waves; at 4
+// (128 VGPRs, a few spilled) two fit, measured 26.0 -> 22.7 ms at 1024^3.
+template <typename R, int N, int PPT> struct P3Waves {
+  static constexpr int value = sizeof(R) == 4 ? (PPT >= 16 && N < 1024 ? 3 : 4) : 1;
 };
 
 // Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
@@ -394,7 +396,7 @@ template <typename R, int PPT> struct P3Waves {
 // offset per point.
 template <typename R, int N, int T, bool W>
 __global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT))
-__attribute__((amdgpu_waves_per_eu(P3Waves<R, StaticPlan<N, W>::PPT>::value))) void k_pass3s(typename CxT<R>::T* a2,
+__attribute__((amdgpu_waves_per_eu(P3Waves<R, N, StaticPlan<N, W>::PPT>::value))) void k_pass3s(typename CxT<R>::T* a2,
                                                                               const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
                                                 int nper, int nrhs, RhsFlags fl) {
