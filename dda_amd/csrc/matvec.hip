@@ -380,8 +380,8 @@ template <bool NT> __device__ __forceinline__ void cstore(double2* p, double2 v)
 // Waves per SIMD the Z pass is compiled for, a register budget: FP32
 // 16-point plans at 3 (168 VGPRs: the tensor loads of several points stay in
 // flight), other FP32 plans at 4, FP64 left to the compiler. At 1024 points
-// the 16-point plan's 512-thread work-groups fit one per CU at 3 waves; at 4
-// (128 VGPRs, a few spilled) two fit, measured 26.0 -> 22.7 ms at 1024^3.
+// the 16-point plan runs 1024-thread work-groups, which fit a CU only at 4
+// waves per SIMD (128 VGPRs, a few spilled).
 template <typename R, int N, int PPT> struct P3Waves {
   static constexpr int value = sizeof(R) == 4 ? (PPT >= 16 && N < 1024 ? 3 : 4) : 1;
 };
@@ -919,7 +919,10 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
 template <typename R, int N, bool W>
 dda_status launch_pass3_static_w(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N, W, 512>();
+  // FP32 1024-point columns: 16 columns (128-byte segments) in a 1024-thread
+  // work-group at 128 VGPRs, the CU's whole register file (17.5 ms at 1024^3,
+  // against 22.7 ms for two 512-thread work-groups of 8 columns)
+  constexpr int T = static_cols<R, N, W, (N == 1024 && sizeof(R) == 4 && W) ? 1024 : 512>();
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const MvDims& d = op->dims;

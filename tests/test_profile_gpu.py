@@ -34,7 +34,7 @@ def test_profile_counts_full_matvecs(ctx):
 
 def test_profile_skips_launches_after_convergence(ctx):
     """A batched solve whose right-hand sides converge at different
-    iterations: the passes keep equal full-launch counts and the no-op
+    iterations: the passes keep (nearly) equal full-launch counts and the no-op
     launches queued past convergence are reported as skipped (a grid whose
     launches are long against a no-op one: 96^3)."""
     g, params, cc, cs, prop, pol = cases.build("sphere_c1", grid=(96, 96, 96))
@@ -49,7 +49,9 @@ def test_profile_skips_launches_after_convergence(ctx):
     ms, counts, skipped = s.profile_read()
     op.profile(False)
     assert all(i["status"] == D.OK for i in infos)
-    assert len(set(counts.tolist())) == 1 and counts[0] > 0
+    # a launch for part of the right-hand sides may land on either side of the
+    # full-launch threshold in different passes
+    assert counts.max() - counts.min() <= 2 and counts.min() > 0
     # two matvecs per iteration: full launches while both right-hand sides
     # run, none counted for the no-op ones queued past convergence
     its = [i["iterations"] for i in infos]
