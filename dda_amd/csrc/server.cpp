@@ -47,6 +47,7 @@ struct Job {
   dda_job_result res{};
   std::string error;  // message of a failed call in the worker
   bool done = false;
+  bool claimed = false;  // a dda_server_wait call holds this job
 };
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
@@ -475,6 +476,11 @@ extern "C" dda_status dda_server_wait(dda_server_t s, uint64_t id, double* pol_o
       return DDA_INVALID_ARG;
     }
     Job* p = it->second.get();
+    if (p->claimed) {  // the first waiter frees the job when it returns
+      dda::set_error("dda_server_wait: another thread is already waiting for this job");
+      return DDA_INVALID_ARG;
+    }
+    p->claimed = true;
     s->cv_done.wait(lk, [&] { return p->done; });
     j = std::move(it->second);
     s->jobs.erase(it);

@@ -354,7 +354,9 @@ dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out);
 /* Copies the job's inputs and returns its id at once. */
 dda_status dda_server_submit(dda_server_t s, const dda_job_desc* job, uint64_t* id);
 /* Blocks until job `id` is done; fills pol_out ([nrhs][3][vec_stride]), info
- * ([nrhs]) and res (all nullable) and forgets the job. Returns its status. */
+ * ([nrhs]) and res (all nullable) and forgets the job. Returns its status.
+ * One thread waits per job: a second wait on a job already being waited for
+ * returns DDA_INVALID_ARG. dda_server_destroy must not overlap a wait. */
 dda_status dda_server_wait(dda_server_t s, uint64_t id, double* pol_out, dda_solve_info* info, dda_job_result* res);
 /* counts = {submitted, completed, cache hits, cache misses, cache evictions};
  * peak_bytes = the largest admitted footprint held at once. */

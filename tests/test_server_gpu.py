@@ -1,6 +1,8 @@
 """Tensor cache and job server: results independent of the mix, admission by
 footprint, cache reuse across geometries on one grid and kd."""
 import math
+import threading
+import time
 
 import numpy as np
 import pytest
@@ -77,6 +79,24 @@ def test_server_result_same_alone_and_in_mix(solver, mixed):
     p_direct, _ = D.SOLVER_CLASSES[solver](op, mixed=mixed).get_pol_vecs(j1.einc, 1e-8)
     assert cases.rel_err(pol_alone, p_direct) <= 1e-6
     ctx.close()
+
+
+def test_server_second_waiter_is_refused():
+    """One thread waits per job: of two concurrent waits on one job, one
+    collects it and the other returns INVALID_ARG (never a freed job)."""
+    j1, _, _ = _small_jobs()
+    srv = D.Server(slots=1, eps=1e-6)
+    jid = S.submit(srv, j1)
+    L = D.lib()
+    first = []
+    t = threading.Thread(target=lambda: first.append(L.dda_server_wait(srv.handle, jid, None, None, None)))
+    t.start()
+    time.sleep(0.05)
+    second = L.dda_server_wait(srv.handle, jid, None, None, None)
+    t.join()
+    srv._shapes.pop(jid)
+    srv.close()
+    assert sorted([first[0], second]) == sorted([D.OK, D.INVALID_ARG])
 
 
 def test_server_admission_and_cache():
