@@ -2234,7 +2234,11 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   void* xv = nullptr;  // dipole vectors for the X passes
   if (op->static_x) {
     DDA_TRY_HIP(hipMalloc(&xv, 2 * 3ull * d.S * csz * nrhs));
-    DDA_TRY_HIP(hipMemsetAsync(xv, 0, 2 * 3ull * d.S * csz * nrhs, st));
+    const hipError_t e = hipMemsetAsync(xv, 0, 2 * 3ull * d.S * csz * nrhs, st);
+    if (e != hipSuccess) {
+      (void)hipFree(xv);
+      DDA_TRY_HIP(e);
+    }
   }
   for (int pass = 0; pass < 5; pass++) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
