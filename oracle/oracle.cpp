@@ -704,13 +704,14 @@ static int dmatrix_impl(const uint32_t* box, const uint32_t* grid, double gridUn
   const int Bx = box[0], By = box[1], Bz = box[2];
   const int Gx = grid[0], Gy = grid[1], Gz = grid[2];
   if (Gx < 2 * Bx - 1 || Gy < 2 * By - 1 || Gz < 2 * Bz - 1) return ST_INVALID;
-  std::vector<C> d2((size_t)Gx * Gy * Gz);  // [gx][gy][gz] Fortran: x fastest
-  auto D2 = [&](int x, int y, int z) -> C& { return d2[x + (size_t)Gx * (y + (size_t)Gy * z)]; };
-  std::vector<C> slice((size_t)Gz * Gy), slice_tr((size_t)Gy * Gz);
-  Fft<ld> fx(Gx), fy(Gy), fz(Gz);
   const ld N = (ld)Gx * Gy * Gz;
+  // the six components are independent: one thread each, with its own buffers
+#pragma omp parallel for schedule(dynamic, 1)
   for (int comp = 0; comp < 6; comp++) {
-    std::fill(d2.begin(), d2.end(), C(0));
+    std::vector<C> d2((size_t)Gx * Gy * Gz, C(0));  // [gx][gy][gz] Fortran: x fastest
+    auto D2 = [&](int x, int y, int z) -> C& { return d2[x + (size_t)Gx * (y + (size_t)Gy * z)]; };
+    std::vector<C> slice((size_t)Gz * Gy), slice_tr((size_t)Gy * Gz);
+    Fft<ld> fx(Gx), fy(Gy), fz(Gz);
     for (int k = 0; k < Gz; k++) {
       int kcor = k > Gz / 2 ? k - Gz : k;
       for (int j = 1 - By; j < By; j++)

@@ -35,6 +35,9 @@ def _geom(name):
         return P.cylinder(9, 3, 1.7 + 0.05j, d), 5e-6
     if name == "sphere_c1":  # 40 dipoles across, lambda 6 um, d 0.5 um (33,552 dipoles)
         return P.sphere(20.0, 1.5 + 0.1j, d), 6e-6
+    if name == "sphere_c2":  # 128 dipoles across, |m| kd = 0.5 (1,099,136 dipoles, 256^3 grid)
+        m = 1.5 + 0.1j
+        return P.sphere(64.0, m, d), 2 * math.pi * d * abs(m) / 0.5
     raise KeyError(name)
 
 

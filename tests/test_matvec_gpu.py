@@ -3,6 +3,8 @@
 The oracle here is the long-double CPU restatement of MatVecCpu
 (DDA/MatVecCpu.cpp:67-150) with the D matrix of DDA/DMatrixCpu.cpp:133-205.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -116,6 +118,26 @@ def test_matvec_c1_full_size(ctx):
     y_ref = orc.matvec(x, precision=64)
     y64 = D.MatVecGpu(_ops(ctx, g, params, cs, 64)).apply(x)
     assert cases.rel_err(y64, y_ref) <= FP64_TOL
+    y32 = D.MatVecGpu(_ops(ctx, g, params, cs, 32)).apply(x)
+    assert cases.rel_err(y32, y_ref) <= FP32_TOL
+
+
+def test_matvec_c2_full_size(ctx):
+    """Sphere 128 dipoles across (1,099,136 dipoles), 256^3 grid: FP64 (both
+    conj flags) and FP32 against the oracle with its own FP64 tensor."""
+    import oracle as O
+    g, params, cc, cs, prop, pol = cases.build("sphere_c2")
+    assert params.nv == 1099136 and params.grid == (256, 256, 256)
+    O.set_threads(min(16, os.cpu_count() or 1))
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params)
+    mv64 = D.MatVecGpu(_ops(ctx, g, params, cs, 64))
+    y_ref = orc.matvec(x)
+    print("c2 fp64", flush=True)
+    assert cases.rel_err(mv64.apply(x), y_ref) <= FP64_TOL
+    print("c2 fp64 conj", flush=True)
+    assert cases.rel_err(mv64.apply(x, conj=True), orc.matvec(x, conj=True)) <= FP64_TOL
+    print("c2 fp32", flush=True)
     y32 = D.MatVecGpu(_ops(ctx, g, params, cs, 32)).apply(x)
     assert cases.rel_err(y32, y_ref) <= FP32_TOL
 

@@ -54,6 +54,29 @@ def test_mixed_c1_to_1e8(ctx):
     assert abs(P.abs_cross(params, p, cc) / P.abs_cross(params, p64, cc) - 1) < 1e-7
 
 
+def test_mixed_c2_block_cext(ctx):
+    """Sphere 128 dipoles across, 256^3 grid: both polarisations as one 2-RHS
+    block, mixed BiCGStab to 1e-6; Cext and Cabs within 1e-5 of FP64 QMR-CS
+    solved to 1e-8."""
+    g, params, cc1, cs1, prop, pol1 = cases.build("sphere_c2")
+    _, _, cc2, cs2, _, pol2 = cases.build("sphere_c2", pol_index=2)
+    einc = np.array([P.einc_plane(params, prop, pol1), P.einc_plane(params, prop, pol2)])
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=64, grid=params.grid)
+    op.set_couple_constants_batch(np.array([cs1, cs2]))
+    print("c2 fp64 qmr", flush=True)
+    p64, i64 = D.GpuQmrCs(op).get_pol_vecs(einc, 1e-8)
+    print("c2 mixed bicgstab", [i["iterations"] for i in i64], flush=True)
+    p, infos = D.GpuBicgStab(op, mixed=True).get_pol_vecs(einc, 1e-6)
+    print([(i["iterations"], i["true_rel_residual"]) for i in infos], flush=True)
+    for r, cc in enumerate((cc1, cc2)):
+        assert i64[r]["status"] == 0 and infos[r]["status"] == 0
+        assert infos[r]["true_rel_residual"] < 1e-6 and infos[r]["refinements"] >= 1
+        c, c64 = P.ext_cross(params, p[r], einc[r]), P.ext_cross(params, p64[r], einc[r])
+        assert abs(c - c64) <= 1e-5 * abs(c64)
+        a, a64 = P.abs_cross(params, p[r], cc), P.abs_cross(params, p64[r], cc)
+        assert abs(a - a64) <= 1e-5 * abs(a64)
+
+
 @pytest.mark.parametrize("kind", ["qmr", "bicg"])
 def test_mixed_near_breakdown_rhs(ctx, kind):
     """An incident wave with |b^T b| / |b|^2 ~ 6e-6 (ellipsoid 32x24x16, the
