@@ -142,6 +142,33 @@ def test_matvec_c2_full_size(ctx):
     assert cases.rel_err(y32, y_ref) <= FP32_TOL
 
 
+def test_matvec_c5_full_size(ctx):
+    """Sphere 512 dipoles across (70,277,000 dipoles), 1024^3 grid, device
+    vectors: the FP32 matvec within 1e-5 of the FP64 one (the 1024-point
+    kernels of both precisions against each other; the oracle is too slow at
+    this size), padding zero."""
+    import math
+    m, d = 1.6 + 0.05j, 1e-7
+    g = P.sphere(256.0, m, d)
+    params = P.DDAParams(g, 2 * math.pi * d * abs(m) / 0.5)
+    assert params.nv == 70277000 and params.grid == (1024, 1024, 1024)
+    prop, pol1, _ = P.beam_pols((0, 0, 1))
+    cc, cs = P.couple_constants(params, prop, pol1)
+    x = P.random_dipole_vector(params)
+    ys = {}
+    for prec in (64, 32):
+        print("c5 precision", prec, flush=True)
+        op = _ops(ctx, g, params, cs, prec)
+        xd, yd = D.DeviceVector(ctx, params.vec_size, prec), D.DeviceVector(ctx, params.vec_size, prec)
+        xd.upload(x)
+        op.apply(xd, yd)
+        ys[prec] = yd.download()
+        for o in (op, xd, yd):
+            o.close()
+    assert np.all(ys[32][params.nv:params.vec_stride] == 0)
+    assert cases.rel_err(ys[32].astype(np.complex128), ys[64]) <= FP32_TOL
+
+
 def test_matvec_complex_symmetry_large(ctx):
     """Size-independent property at a 128^3 grid: x^T (A y) = y^T (A x)."""
     g = P.sphere(32.0, 1.5 + 0.1j, 0.5e-6)
