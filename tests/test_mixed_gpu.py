@@ -75,6 +75,15 @@ def test_mixed_c2_block_cext(ctx):
         assert abs(c - c64) <= 1e-5 * abs(c64)
         a, a64 = P.abs_cross(params, p[r], cc), P.abs_cross(params, p64[r], cc)
         assert abs(a - a64) <= 1e-5 * abs(a64)
+        # Mie theory for the volume-equivalent sphere: the discretisation error
+        # at |m| kd = 0.5 and 128 dipoles across is ~0.1%; the bound is 0.5%
+        import math
+        import oracle as O
+        a_eq = (3 * params.nv / (4 * math.pi)) ** (1 / 3) * g.grid_unit
+        qext, qsca, _, _ = O.mie(params.wave_num * a_eq, 1.5 + 0.1j)
+        area = math.pi * a_eq ** 2
+        assert abs(c / (qext * area) - 1) < 5e-3
+        assert abs(a / ((qext - qsca) * area) - 1) < 5e-3
 
 
 @pytest.mark.parametrize("kind", ["qmr", "bicg"])
