@@ -272,8 +272,8 @@ def main():
                          "K FP64 matvecs; prints per-pass times (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU slab "
-              "decomposition is a design note only)" % args.gpus, file=sys.stderr)
+        print("bench.py: the solvers run on one GPU; --gpus %d is not supported (the multi-GPU slab "
+              "decomposition is not implemented)" % args.gpus, file=sys.stderr)
         return 2
 
     import numpy as np
