@@ -3,10 +3,12 @@
 
 Workload (the single-GPU configuration the metric is quoted on): a sphere 128
 dipoles across (1,099,136 dipoles), m = 1.5 + 0.1i, |m| kd = 0.5, FFT grid
-256^3, plane wave along z, both polarisations, each solved separately with
-QMR-CS to |r| / |b| < 1e-6. One step = the two solves. Inputs are synthetic
-(geometry and incident field generated here) and resident on the GPU when the
-timed region starts apart from the incident field (6 MB per solve, included).
+256^3, plane wave along z, both polarisations solved to |r| / |b| < 1e-6. One
+step = the two solves. Runs: FP64 QMR-CS (the baseline), FP64 BiCGStab, mixed
+QMR-CS, mixed BiCGStab, and the two mixed ones with both polarisations as one
+2-RHS block; value = the last run. Inputs are synthetic (geometry and incident
+fields generated here) and resident on the GPU when the timed region starts;
+the polarisations stay there.
 
 Printed JSON line: value = solves/s; `roofline` for the dominant matvec kernel
 from HIP events recorded around every launch in the timed region; the
@@ -112,6 +114,71 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np, nrhs=1):
         out[name + "_pass_avg_us"] = [1e3 * ms[i] / max(cnt[i], 1) for i in range(5)]
     print(json.dumps(out))
     return 0
+
+
+def c3_bench(args, D, P, np, ctx):
+    """Batched right-hand sides: ellipsoid with semi-axes 32x24x16 dipoles,
+    m = 1.33+0.01i, |m| kd = 0.6, 64 seeded directions x 2 polarisations = 128
+    fields sharing one tensor. Mixed block QMR-CS (one 128-RHS block) against
+    FP64 QMR-CS solving the same fields one at a time, both to args.eps on
+    device-resident fields; Cext of every field compared."""
+    m, d = 1.33 + 0.01j, 1e-7
+    g = P.ellipsoid(32, 24, 16, m, d)
+    params = P.DDAParams(g, 2 * math.pi * d * abs(m) / 0.6)
+    einc, ccs, props, pols = P.incident_batch(params, P.directions(64, 2024))
+    k = len(ccs)
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=64, grid=params.grid)
+    op.set_couple_constants(ccs[0])
+    if not args.no_tune:
+        op.tune(1)
+    vs = op.vec_size
+    e1, p1 = D.DeviceVector(ctx, vs, 64), D.DeviceVector(ctx, vs, 64)
+    s64 = D.GpuQmrCs(op)
+    e1.upload(einc[0])
+    s64.solve_device(e1, p1, 1, args.eps)  # warm-up
+    ctx.synchronize()
+    p64, it64, el64 = [], [], 0.0
+    for r in range(k):
+        e1.upload(einc[r])  # outside the timed span
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        s64.set_couple_constants(ccs[r])
+        inf = s64.solve_device(e1, p1, 1, args.eps)[0]
+        ctx.synchronize()
+        el64 += time.perf_counter() - t0
+        p64.append(p1.download())
+        it64.append(int(inf["iterations"]))
+    s64.close()
+    e1.close()
+    p1.close()
+    ek, pk = D.DeviceVector(ctx, k * vs, 64), D.DeviceVector(ctx, k * vs, 64)
+    ek.upload(einc.reshape(-1))
+    sb = D.GpuQmrCs(op, mixed=True)
+    op.set_couple_constants_batch(ccs)
+    sb.solve_device(ek, pk, k, args.eps)  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    op.set_couple_constants_batch(ccs)
+    infos = sb.solve_device(ek, pk, k, args.eps)
+    ctx.synchronize()
+    elb = time.perf_counter() - t0
+    pb = pk.download().reshape(k, vs)
+    sb.close()
+    ek.close()
+    pk.close()
+    op.close()
+    cerr = max(abs(P.ext_cross(params, pb[r], einc[r]) / P.ext_cross(params, p64[r], einc[r]) - 1) for r in range(k))
+    return {"workload": "ellipsoid semi-axes 32x24x16 dipoles (nv=%d), m=1.33+0.01i, |m|kd=0.6, grid %s, 64 seeded "
+                        "directions x 2 polarisations = %d right-hand sides, to %g" % (
+                            params.nv, "x".join(map(str, params.grid)), k, args.eps),
+            "fp64_qmr_one_at_a_time": {"solves_per_s": k / el64, "seconds": el64,
+                                       "iterations": [min(it64), max(it64)]},
+            "mixed_qmr_block": {"solves_per_s": k / elb, "seconds": elb,
+                                "iterations": [min(int(i["iterations"]) for i in infos),
+                                               max(int(i["iterations"]) for i in infos)],
+                                "ok": sum(1 for i in infos if i["status"] == D.OK)},
+            "vs_fp64_qmr": el64 / elb,
+            "cext_max_rel_err": cerr}
 
 
 def large_bench(args, D, P, np):
@@ -254,6 +321,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 128-RHS ellipsoid batch figure")
     ap.add_argument("--host-step", action="store_true",
                     help="also time one step through the host-array path (PCIe copies included)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default kernel variants)")
@@ -505,6 +573,13 @@ def main():
         except Exception as e:
             orient = {"error": str(e)}
 
+    c3 = None
+    if not args.no_c3:
+        try:
+            c3 = c3_bench(args, D, P, np, ctx)
+        except Exception as e:  # reported, not fatal
+            c3 = {"error": str(e)}
+
     serving = None
     if args.serve_jobs > 0:
         try:
@@ -571,6 +646,7 @@ def main():
         "device": info,
         "cpu_baseline": cpu,
         "orientation_batch": orient,
+        "c3_batch": c3,
         "serving": serving,
     }
     print(json.dumps(_plain(line)))
