@@ -119,9 +119,10 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np, nrhs=1):
 def c3_bench(args, D, P, np, ctx):
     """Batched right-hand sides: ellipsoid with semi-axes 32x24x16 dipoles,
     m = 1.33+0.01i, |m| kd = 0.6, 64 seeded directions x 2 polarisations = 128
-    fields sharing one tensor. Mixed block QMR-CS (one 128-RHS block) against
-    FP64 QMR-CS solving the same fields one at a time, both to args.eps on
-    device-resident fields; Cext of every field compared."""
+    fields sharing one tensor. Mixed block QMR-CS and mixed block BiCGStab (one
+    128-RHS block each) against FP64 QMR-CS solving the same fields one at a
+    time, all to args.eps on device-resident fields; Cext of every field
+    compared."""
     m, d = 1.33 + 0.01j, 1e-7
     g = P.ellipsoid(32, 24, 16, m, d)
     params = P.DDAParams(g, 2 * math.pi * d * abs(m) / 0.6)
@@ -153,32 +154,32 @@ def c3_bench(args, D, P, np, ctx):
     p1.close()
     ek, pk = D.DeviceVector(ctx, k * vs, 64), D.DeviceVector(ctx, k * vs, 64)
     ek.upload(einc.reshape(-1))
-    sb = D.GpuQmrCs(op, mixed=True)
-    op.set_couple_constants_batch(ccs)
-    sb.solve_device(ek, pk, k, args.eps)  # warm-up
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    op.set_couple_constants_batch(ccs)
-    infos = sb.solve_device(ek, pk, k, args.eps)
-    ctx.synchronize()
-    elb = time.perf_counter() - t0
-    pb = pk.download().reshape(k, vs)
-    sb.close()
+    out = {"workload": "ellipsoid semi-axes 32x24x16 dipoles (nv=%d), m=1.33+0.01i, |m|kd=0.6, grid %s, 64 seeded "
+                       "directions x 2 polarisations = %d right-hand sides, to %g" % (
+                           params.nv, "x".join(map(str, params.grid)), k, args.eps),
+           "fp64_qmr_one_at_a_time": {"solves_per_s": k / el64, "seconds": el64,
+                                      "iterations": [min(it64), max(it64)]}}
+    for name, cls in (("mixed_qmr_block", D.GpuQmrCs), ("mixed_bicgstab_block", D.GpuBicgStab)):
+        sb = cls(op, mixed=True)
+        op.set_couple_constants_batch(ccs)
+        sb.solve_device(ek, pk, k, args.eps)  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        op.set_couple_constants_batch(ccs)
+        infos = sb.solve_device(ek, pk, k, args.eps)
+        ctx.synchronize()
+        elb = time.perf_counter() - t0
+        pb = pk.download().reshape(k, vs)
+        sb.close()
+        cerr = max(abs(P.ext_cross(params, pb[r], einc[r]) / P.ext_cross(params, p64[r], einc[r]) - 1)
+                   for r in range(k))
+        out[name] = {"solves_per_s": k / elb, "seconds": elb, "vs_fp64_qmr": el64 / elb,
+                     "iterations": [min(int(i["iterations"]) for i in infos), max(int(i["iterations"]) for i in infos)],
+                     "ok": sum(1 for i in infos if i["status"] == D.OK), "cext_max_rel_err": cerr}
     ek.close()
     pk.close()
     op.close()
-    cerr = max(abs(P.ext_cross(params, pb[r], einc[r]) / P.ext_cross(params, p64[r], einc[r]) - 1) for r in range(k))
-    return {"workload": "ellipsoid semi-axes 32x24x16 dipoles (nv=%d), m=1.33+0.01i, |m|kd=0.6, grid %s, 64 seeded "
-                        "directions x 2 polarisations = %d right-hand sides, to %g" % (
-                            params.nv, "x".join(map(str, params.grid)), k, args.eps),
-            "fp64_qmr_one_at_a_time": {"solves_per_s": k / el64, "seconds": el64,
-                                       "iterations": [min(it64), max(it64)]},
-            "mixed_qmr_block": {"solves_per_s": k / elb, "seconds": elb,
-                                "iterations": [min(int(i["iterations"]) for i in infos),
-                                               max(int(i["iterations"]) for i in infos)],
-                                "ok": sum(1 for i in infos if i["status"] == D.OK)},
-            "vs_fp64_qmr": el64 / elb,
-            "cext_max_rel_err": cerr}
+    return out
 
 
 def large_bench(args, D, P, np):
