@@ -49,6 +49,8 @@ def lib():
         L.orc_geometry.restype = ctypes.c_long
         L.orc_beam_pols.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_cc.argtypes = [ctypes.c_int, ctypes.c_uint32, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_cc_planted.argtypes = [ctypes.c_int, ctypes.c_uint32, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_einc_plane.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, _dp]
         L.orc_dmatrix.argtypes = [_u32p, _u32p, ctypes.c_double, ctypes.c_double, _dp]
@@ -58,6 +60,8 @@ def lib():
         L.orc_matvec_direct.argtypes = [ctypes.POINTER(_Prob), ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.orc_solve.argtypes = [ctypes.POINTER(_Prob), ctypes.c_int, _dp, ctypes.c_double, ctypes.c_uint64, _dp,
                                 _dp, _u64p, _dp, ctypes.c_uint64]
+        L.orc_solve_ex.argtypes = [ctypes.POINTER(_Prob), ctypes.c_int, _dp, ctypes.c_double, ctypes.c_uint64, _dp,
+                                   _dp, _u64p, _dp, ctypes.c_uint64, ctypes.c_uint64]
         L.orc_ext_cross_sum.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _dp, _dp, ctypes.c_double]
         L.orc_ext_cross_sum.restype = ctypes.c_double
         L.orc_abs_cross.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _dp, _u8p, ctypes.c_uint32, _dp, ctypes.c_double]
@@ -122,8 +126,10 @@ def beam_pols(prop):
     return out
 
 
-def couple_constants(kind, m, grid_unit, kd, prop, incpol):
-    """PolarizabilityDescription.cpp:65-110 -> (cc, cc_sqrt) as [nmat,3] complex."""
+def couple_constants(kind, m, grid_unit, kd, prop, incpol, planted=None):
+    """PolarizabilityDescription.cpp:65-110 -> (cc, cc_sqrt) as [nmat,3] complex.
+    planted = (b1, b2, b3, s_factor) replaces the LDR constants and scales S: a
+    mistake planted by a test to show that its check rejects it."""
     kinds = {"cm": 0, "ldr": 1, "ldr-avgpol": 2}
     m = np.asarray(m, np.complex128)
     if m.ndim == 0:
@@ -135,8 +141,13 @@ def couple_constants(kind, m, grid_unit, kd, prop, incpol):
     cs = np.zeros_like(m)
     pr = np.asarray(prop, np.float64)
     ip = np.asarray(incpol, np.float64)
-    lib().orc_cc(kinds[kind], m.shape[0], _p(m.view(np.float64), _dp), grid_unit, kd,
-                 _p(pr, _dp), _p(ip, _dp), _p(cc.view(np.float64), _dp), _p(cs.view(np.float64), _dp))
+    if planted is None:
+        lib().orc_cc(kinds[kind], m.shape[0], _p(m.view(np.float64), _dp), grid_unit, kd,
+                     _p(pr, _dp), _p(ip, _dp), _p(cc.view(np.float64), _dp), _p(cs.view(np.float64), _dp))
+    else:
+        lib().orc_cc_planted(kinds[kind], m.shape[0], _p(m.view(np.float64), _dp), grid_unit, kd,
+                             _p(pr, _dp), _p(ip, _dp), *[float(v) for v in planted],
+                             _p(cc.view(np.float64), _dp), _p(cs.view(np.float64), _dp))
     return cc, cs
 
 
@@ -214,8 +225,9 @@ class Problem:
                                 _p(x.view(np.float64), _dp), _p(y.view(np.float64), _dp), int(conj))
         return y
 
-    def solve(self, kind, einc, eps, max_iter=0, start=None, max_iter_sample=0):
+    def solve(self, kind, einc, eps, max_iter=0, start=None, max_iter_sample=0, max_res_increase=0):
         """IterativeSolverBase.cpp:40-103 + the CPU solver `kind`.
+        max_res_increase = 0 takes the solver's default.
         Returns (status, pvec, iterations, residual history)."""
         einc = _cd(einc)
         pvec = np.zeros(3 * self.vec_stride, np.complex128)
@@ -223,10 +235,10 @@ class Problem:
         mi = max_iter if max_iter else 3 * int(np.prod(self.box.astype(np.int64)))
         hist = np.full(min(mi, 100000) + 1, np.nan)
         st = None if start is None else _cd(start)
-        status = lib().orc_solve(ctypes.byref(self._s), SOLVERS[kind], _p(einc.view(np.float64), _dp), eps,
-                                 max_iter, None if st is None else _p(st.view(np.float64), _dp),
-                                 _p(pvec.view(np.float64), _dp), ctypes.byref(iters),
-                                 _p(hist, _dp) if mi <= 100000 else None, max_iter_sample)
+        status = lib().orc_solve_ex(ctypes.byref(self._s), SOLVERS[kind], _p(einc.view(np.float64), _dp), eps,
+                                    max_iter, None if st is None else _p(st.view(np.float64), _dp),
+                                    _p(pvec.view(np.float64), _dp), ctypes.byref(iters),
+                                    _p(hist, _dp) if mi <= 100000 else None, max_iter_sample, max_res_increase)
         n = int(iters.value)
         return status, pvec, n, hist[: n + 1]
 

@@ -29,15 +29,23 @@
 //   orc_matvec         dense O(nv^2) direct sum (orc_matvec_direct), grid-size
 //                      invariance, complex symmetry x^T A y = y^T A x
 //   orc_matvec_direct  Green tensor closed form checked at hand-computed points
-//   orc_solve          numpy.linalg.solve on the dense matrix; all four solvers agree
-//   orc_cc             Clausius-Mossotti closed form; LDR -> CM as kd -> 0
+//   orc_solve          numpy.linalg.solve on the dense matrix; all four solvers agree;
+//                      BREAKDOWN forced by a right-hand side with b^T b = 0,
+//                      STAGNATION checked against the returned residual history
+//   orc_cc             Clausius-Mossotti closed form; LDR -> CM as kd -> 0; the
+//                      finite-kd terms (b1..b3, S) by the lattice sphere's
+//                      absorption against Mie at oblique incidence, which
+//                      planted wrong constants miss; LDR-avgpol is the mean of
+//                      1/alpha over both polarisations
 //   orc_mie            optical theorem / known Qext values (textbook)
 //   orc_ext_cross_*    the two Cext formulas agree (optical theorem)
 //   orc_abs_cross,     energy conservation Cext = Cabs + Csca (Csca by quadrature
 //   orc_calc_field     of |E_far|^2); Cabs = 0 for real m
 //   orc_jones          integral of sum |J_ij|^2 = Csca(pol 1) + Csca(pol 2); zero
 //                      off-diagonals on the lattice sphere's mirror planes; forward
-//                      amplitude gives Cext
+//                      amplitude gives Cext; symmetric = 1 from one solve equals
+//                      the two-solve result on a chiral quarter-turn-symmetric
+//                      target, where planted sign or axis mistakes differ
 //   orc_beam_pols,     orthonormal frame, pol1 horizontal, pol2 = pol1 x prop;
 //   orc_einc_plane     unit amplitude, phase k d per step along prop
 // Exact iteration counts and residual histories of the reference (built with
@@ -642,10 +650,13 @@ void orc_beam_pols(const double* prop_in, double* prop_out, double* pol1, double
 // material, PolarizabilityDescription.cpp:65-110 and DDAParams.cpp:315-331.
 // kind: 0 = CM, 1 = LDR, 2 = LDR with avgpol. m: [nmat][3] complex refractive
 // index per axis. prop/incpol: unit vectors in the particle frame.
-void orc_cc(int kind, uint32_t nmat, const double* m, double gridUnit, double kd,
-            const double* prop, const double* incpol, double* cc_out, double* cc_sqrt_out) {
+// orc_cc_planted takes the three LDR constants and a factor on S as arguments,
+// so that tests can plant a wrong constant and show that their physical checks
+// reject it; orc_cc passes the reference's values.
+void orc_cc_planted(int kind, uint32_t nmat, const double* m, double gridUnit, double kd, const double* prop,
+                    const double* incpol, double b1, double b2, double b3, double s_factor, double* cc_out,
+                    double* cc_sqrt_out) {
   typedef std::complex<double> C;
-  const double b1 = 1.8915316, b2 = -0.1648469, b3 = 1.7700004;  // FPConst.hpp:48-50
   double vol = gridUnit * gridUnit * gridUnit;
   double temp = vol / (4 * (double)kPi);
   for (uint32_t i = 0; i < nmat; i++)
@@ -663,6 +674,7 @@ void orc_cc(int kind, uint32_t nmat, const double* m, double gridUnit, double kd
         } else {
           S = p2[0] * incpol[0] * incpol[0] + p2[1] * incpol[1] * incpol[1] + p2[2] * incpol[2] * incpol[2];
         }
+        S *= s_factor;
         t1 += (b1 + (b2 + b3 * S) * m2) * kd * kd;
         cc = unc / (C(1) - (unc / vol) * t1);
       }
@@ -670,6 +682,13 @@ void orc_cc(int kind, uint32_t nmat, const double* m, double gridUnit, double kd
       cc_out[2 * (3 * i + j)] = cc.real(); cc_out[2 * (3 * i + j) + 1] = cc.imag();
       cc_sqrt_out[2 * (3 * i + j)] = cs.real(); cc_sqrt_out[2 * (3 * i + j) + 1] = cs.imag();
     }
+}
+
+void orc_cc(int kind, uint32_t nmat, const double* m, double gridUnit, double kd, const double* prop,
+            const double* incpol, double* cc_out, double* cc_sqrt_out) {
+  // ldr_b1..b3, FPConst.hpp:48-50 (doi:10.1086/172396)
+  orc_cc_planted(kind, nmat, m, gridUnit, kd, prop, incpol, 1.8915316, -0.1648469, 1.7700004, 1.0, cc_out,
+                 cc_sqrt_out);
 }
 
 // Plane wave incident field, Beam.cpp:110-118: E_i = exp(i k r_i . prop) pol,
@@ -835,9 +854,11 @@ int orc_matvec_direct(const orc_prob* p, double gridUnit, double waveNum, const 
 // resid_out (nullable, length max_iter + 1) receives residScale * |r|^2 after
 // each iteration (index 0 = initial). max_iter = 0 selects the reference default
 // 3 * Bx * By * Bz (DDAMain.cpp:640-644).
-int orc_solve(const orc_prob* p, int kind, const double* einc, double eps, uint64_t max_iter,
-              const double* start, double* pvec_out, uint64_t* iters_out, double* resid_out,
-              uint64_t max_iter_sample) {
+// max_res_increase = 0 selects the solver's default (QmrCs.cpp:39,
+// BicgStab.cpp:51, Cgnr.cpp:34, BicgCs.cpp:48).
+int orc_solve_ex(const orc_prob* p, int kind, const double* einc, double eps, uint64_t max_iter,
+                 const double* start, double* pvec_out, uint64_t* iters_out, double* resid_out,
+                 uint64_t max_iter_sample, uint64_t max_res_increase) {
   Prob g = toProb(p);
   typedef double T;
   typedef std::complex<T> C;
@@ -852,7 +873,7 @@ int orc_solve(const orc_prob* p, int kind, const double* einc, double eps, uint6
   s.inprodR = ip;
   s.epsB = (T)(eps * eps / s.residScale);
   size_t count = 0, counter = 0;
-  size_t maxResIncrease = maxResIncreaseFor(kind);
+  size_t maxResIncrease = max_res_increase ? max_res_increase : maxResIncreaseFor(kind);
   if (resid_out) resid_out[0] = s.residScale * ip;
   s.init();
   int status = ST_OK;
@@ -876,6 +897,12 @@ int orc_solve(const orc_prob* p, int kind, const double* einc, double eps, uint6
       pvec_out[2 * (i + c * g.vec_stride) + 1] = v.imag();
     }
   return status;
+}
+
+int orc_solve(const orc_prob* p, int kind, const double* einc, double eps, uint64_t max_iter,
+              const double* start, double* pvec_out, uint64_t* iters_out, double* resid_out,
+              uint64_t max_iter_sample) {
+  return orc_solve_ex(p, kind, einc, eps, max_iter, start, pvec_out, iters_out, resid_out, max_iter_sample, 0);
 }
 
 // Cext = 4 pi k sum Im(p_i . conj(E_i)), Beam.cpp:58-68.
@@ -949,7 +976,9 @@ double orc_ext_cross_forward(uint32_t nv, uint32_t vec_stride, const double* pve
 // Jones matrices on a (theta, phi) grid, FarFieldCalc.cpp:119-174 and
 // JonesCalculus.cpp:74-86. p1 solves polarisation 1, p2 polarisation 2 (for a
 // symmetric target the reference passes p1 with rotated axes; callers do that
-// by passing the same vector and `symmetric` = 1). out: [count][2][2] complex.
+// by passing the same vector and `symmetric` = 1; symmetric = 2 and 3 are
+// mistakes that tests plant to show their checks reject them). out:
+// [count][2][2] complex.
 void orc_jones(uint32_t nv, uint32_t vec_stride, const double* p1, const double* p2, int symmetric,
                const uint32_t* pos, const uint32_t* box, double gridUnit, double waveNum, const double* prop,
                const double* pol1, const double* pol2, uint32_t ntheta, double theta0, double dtheta,
@@ -976,8 +1005,12 @@ void orc_jones(uint32_t nv, uint32_t vec_stride, const double* p1, const double*
     C perp1, par1, perp2, par2;
     // eField1: incPolX = pol2, incPolY = pol1 (FarFieldCalc.cpp:165)
     field(p1, pol2, pol1, false, i, perp1, par1);
-    if (symmetric)  // FarFieldCalc.cpp:167-168: X = pol1, Y = -pol2
+    if (symmetric == 1)  // FarFieldCalc.cpp:167-168: X = pol1, Y = -pol2
       field(p1, pol1, pol2, true, i, perp2, par2);
+    else if (symmetric == 2)  // a mistake planted by tests: the minus on pol2 dropped
+      field(p1, pol1, pol2, false, i, perp2, par2);
+    else if (symmetric == 3)  // a mistake planted by tests: the rotation's axes swapped
+      field(p1, pol2, pol1, true, i, perp2, par2);
     else
       field(p2, pol2, pol1, false, i, perp2, par2);
     double ph = phi0 + (double)(i % nphi) * dphi;

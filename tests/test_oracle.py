@@ -397,3 +397,165 @@ def test_beam_pols_and_plane_wave():
     assert np.allclose(np.abs(E[:, 1]), 1, atol=1e-15) and np.all(E[:, 0] == 0) and np.all(E[:, 2] == 0)
     z = pos[:, 2].astype(float) - (box[2] - 1) / 2.0
     assert np.allclose(E[:, 1], np.exp(1j * k * d * z), atol=1e-14)
+
+
+# The LDR constants of FPConst.hpp:48-50 (doi:10.1086/172396), used only to
+# plant variants below; the oracle's own values are not read from here.
+_LDR_B = (1.8915316, -0.1648469, 1.7700004)
+
+
+def _sphere_cabs_error(nd, m, mkd, prop_dir, planted=None, kind="ldr"):
+    """Mean relative error of Cabs (both polarisations) of the lattice sphere nd
+    dipoles across against Mie theory for the volume-equivalent sphere."""
+    d = 1.0
+    kd = mkd / abs(m)
+    k = kd / d
+    pos, box = O.geometry("sphere", float(nd))
+    grid = np.array([O.fft_fit(2 * int(b)) for b in box], np.uint32)
+    prop, pol1, pol2 = O.beam_pols(np.asarray(prop_dir, float) / np.linalg.norm(prop_dir))
+    a_eq = (3 * len(pos) / (4 * math.pi)) ** (1 / 3) * d
+    qext, qsca, _, _ = O.mie(k * a_eq, m)
+    cabs_mie = (qext - qsca) * math.pi * a_eq ** 2
+    dmat, cabs = None, []
+    for pol in (pol1, pol2):
+        cc, cs = O.couple_constants(kind, m, d, kd, prop, pol, planted=planted)
+        P = O.Problem(pos, None, box, grid, cs, d, k, dmat=dmat)
+        dmat = P.dmat
+        e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol)
+        st, pv, _, _ = P.solve("qmr", e, 1e-9, max_iter=5000)
+        assert st == O.ST_OK
+        cabs.append(P.abs_cross(pv, cc))
+    return abs(np.mean(cabs) / cabs_mie - 1)
+
+
+def test_ldr_finite_kd_terms_against_mie_at_oblique_incidence():
+    """Pins the O((kd)^2) part of the LDR polarisability (b1, b2, b3 and S =
+    sum prop_i^2 pol_i^2, PolarizabilityDescription.cpp:85-110) by physics: the
+    LDR makes the dipole lattice propagate light with the medium's own index,
+    so the absorption of a lattice sphere follows Mie theory closely. Along
+    (1,1,1)/sqrt(3), S = 1/3 for every polarisation. The true constants meet a
+    0.7% tolerance (measured 0.26%); each planted mistake -- S dropped, b3, b2 or
+    b1 with the wrong sign -- misses it (measured 1.4% to 4.4%)."""
+    O.set_threads(8)
+    case = (12, 1.5 + 0.1j, 0.5, (1, 1, 1))
+    tol = 7e-3
+    assert _sphere_cabs_error(*case) < tol
+    b1, b2, b3 = _LDR_B
+    for planted in ((b1, b2, b3, 0.0), (b1, b2, -b3, 1.0), (b1, -b2, b3, 1.0), (-b1, b2, b3, 1.0)):
+        assert _sphere_cabs_error(*case, planted=planted) > tol, planted
+    # Clausius-Mossotti (no lattice correction at all) misses it too
+    assert _sphere_cabs_error(*case, kind="cm") > tol
+    O.set_threads(1)
+
+
+def test_ldr_avgpol_is_the_polarisation_mean():
+    """1/alpha is linear in S, and LDR-avgpol's S = (1 - sum prop_i^4) / 2 is the
+    mean of S = sum prop_i^2 pol_i^2 over two orthogonal polarisations across
+    prop (sum over pol of pol_i^2 = 1 - prop_i^2), so 1/alpha_avgpol is the mean
+    of 1/alpha over polarisations 1 and 2, for any direction."""
+    rng = np.random.default_rng(4)
+    m = [1.33 + 0.01j, 2.0 + 0.4j]
+    for _ in range(4):
+        prop, p1, p2 = O.beam_pols(rng.normal(size=3))
+        a1, _ = O.couple_constants("ldr", m, 1.0, 0.6, prop, p1)
+        a2, _ = O.couple_constants("ldr", m, 1.0, 0.6, prop, p2)
+        av, _ = O.couple_constants("ldr-avgpol", m, 1.0, 0.6, prop, p1)
+        np.testing.assert_allclose(1 / av, (1 / a1 + 1 / a2) / 2, rtol=1e-13)
+        # and it does depend on the direction's S (not a constant)
+        assert np.max(np.abs(1 / a1 - 1 / a2)) > 1e-3 * np.max(np.abs(1 / a1))
+
+
+def pinwheel_target():
+    """A chiral dipole set: nine dipoles of a twisted arm, repeated by quarter
+    turns about the z axis through the centre of an 8x8x3 box. It is
+    symmetric under those quarter turns and under no mirror plane."""
+    B = 8
+    arm = [(4, 3, 0), (5, 3, 0), (6, 3, 0), (7, 3, 0), (7, 4, 1), (7, 5, 1), (6, 6, 2), (4, 4, 0), (5, 4, 1)]
+    pts = set()
+    for (x, y, z) in arm:
+        for _ in range(4):
+            pts.add((x, y, z))
+            x, y = B - 1 - y, x
+    pos = np.array(sorted(pts, key=lambda p: (p[2], p[1], p[0])), np.uint32)
+    return pos, pos.max(axis=0) + 1
+
+
+def test_symmetric_jones_equals_two_solves():
+    """FarFieldCalc.cpp:163-168: for a target symmetric under the quarter turn
+    about the propagation axis that takes polarisation 1 to polarisation 2, the
+    field of polarisation 2 is that of polarisation 1 with the axes (X, Y) =
+    (pol1, -pol2). The chiral pinwheel along z is such a target (LDR gives both
+    polarisations the same polarisability there, S = 0), so the Jones matrices
+    from one solve with symmetric = 1 equal those from both solves to the
+    solver's tolerance. Because the target has no mirror plane, dropping the
+    minus on pol2 (the mirror image) or swapping the two axes -- mistakes
+    planted in the oracle -- gives different matrices."""
+    d, lam, m = 0.5e-6, 3e-6, 1.5 + 0.1j
+    k = 2 * math.pi / lam
+    pos, box = pinwheel_target()
+    g = np.array([O.fft_fit(2 * int(b)) for b in box], np.uint32)
+    prop, pol1, pol2 = O.beam_pols([0, 0, 1])
+    ps = []
+    for pol in (pol1, pol2):
+        cc, cs = O.couple_constants("ldr", m, d, k * d, prop, pol)
+        P = O.Problem(pos, None, box, g, cs, d, k)
+        e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol)
+        st, pvec, _, _ = P.solve("qmr", e, 1e-12, max_iter=3000)
+        assert st == O.ST_OK
+        ps.append(pvec)
+    args = (prop, pol1, pol2, 13, 0.0, math.pi / 12, 12, 0.0, math.pi / 6)
+    J2 = P.jones(ps[0], ps[1], 0, *args)
+    J1 = P.jones(ps[0], None, 1, *args)
+    scale = np.max(np.abs(J2))
+    assert np.max(np.abs(J1 - J2)) <= 1e-9 * scale
+    for planted in (2, 3):
+        Jbad = P.jones(ps[0], None, planted, *args)
+        assert np.max(np.abs(Jbad - J2)) > 1e-2 * scale, planted
+
+
+def test_solver_breakdown_status():
+    """A right-hand side with b^T b = 0 (b != 0: E_x = 1 at one dipole, i at the
+    next) started from x0 = 0 makes BiCG-CS's rho = r^T r zero at its first
+    iteration, which the reference stops on (BicgCs.cpp:68): status BREAKDOWN
+    after 0 iterations, with x still 0."""
+    P, prop, pol1, cc = _problem("sphere", (3.0, 0, 0))
+    e = np.zeros(3 * P.vec_stride, complex)
+    e[0], e[1] = 1.0, 1.0j
+    st, pv, it, _ = P.solve("bicg", e, 1e-8, max_iter=100, start=np.zeros_like(e))
+    assert st == O.ST_BREAKDOWN and it == 0
+    assert np.all(pv == 0)
+
+
+def _stagnation_case():
+    d, lam, m = 0.5e-6, 6e-6, 3.0 + 0.01j
+    k = 2 * math.pi / lam
+    pos, box = O.geometry("sphere", 8.0)
+    g = np.array([O.fft_fit(2 * int(b)) for b in box], np.uint32)
+    prop, pol1, _ = O.beam_pols([0, 0, 1])
+    cc, cs = O.couple_constants("ldr", m, d, k * d, prop, pol1)
+    P = O.Problem(pos, None, box, g, cs, d, k)
+    e = O.einc_plane(P.pos, P.box, P.vec_stride, d, k, prop, pol1)
+    return P, e
+
+
+@pytest.mark.parametrize("kind,mri,it_expected", [("bicg", 1, 6), ("bicg", 2, 12), ("qmr", 1, 27)])
+def test_solver_stagnation_status(kind, mri, it_expected):
+    """IterativeSolverBase.cpp:56-70: the solve stops with "Too many iterations w/o
+    increase" once more than maxResIncrease iterations in a row failed to lower
+    the smallest residual seen. On a high-index sphere (m = 3 + 0.01i, whose
+    residuals are erratic) the stop comes at the first iteration where the
+    residual history returned by the solve has mri + 1 such iterations in a row,
+    well before convergence."""
+    P, e = _stagnation_case()
+    st, _, it, hist = P.solve(kind, e, 1e-10, max_iter=3000, max_res_increase=mri)
+    assert st == O.ST_STAGNATION and it == it_expected
+    best, run = hist[0], 0
+    for i, h in enumerate(hist[1:], 1):
+        run = 0 if h <= best else run + 1
+        best = min(best, h)
+        if run > mri:
+            break
+    assert i == it
+    # the same solve with the default limit goes on (no stagnation that early)
+    st2, _, it2, _ = P.solve(kind, e, 1e-10, max_iter=it + 5)
+    assert st2 != O.ST_STAGNATION
