@@ -33,7 +33,7 @@ SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_
 EXPORTS = [
     "dda_last_error", "dda_ctx_create", "dda_ctx_destroy", "dda_ctx_synchronize", "dda_ctx_info",
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
-    "dda_fft_fit_static",
+    "dda_fft_fit_static", "dda_grid_plan",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_set_cc_sqrt_batch",
     "dda_op_apply", "dda_op_apply_batch", "dda_op_tune",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
@@ -92,7 +92,8 @@ class JobDesc(ctypes.Structure):
 class JobResult(ctypes.Structure):
     _fields_ = [("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("status", ctypes.c_int32), ("packed_rhs", ctypes.c_int32), ("footprint", ctypes.c_size_t),
-                ("cache_hit", ctypes.c_int32), ("iterations", ctypes.c_int32), ("setup_s", ctypes.c_double)]
+                ("cache_hit", ctypes.c_int32), ("iterations", ctypes.c_int32), ("setup_s", ctypes.c_double),
+                ("tensor_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -125,6 +126,7 @@ def lib():
         L.dda_fft_fit_static.argtypes = [u32]
         L.dda_fft_fit_static.restype = u32
         L.dda_fft_fit.restype = u32
+        L.dda_grid_plan.argtypes = [ctypes.POINTER(u32), i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
         L.dda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), i32, pp]
         L.dda_op_destroy.argtypes = [vp]
         L.dda_op_info.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(sz)]
@@ -166,7 +168,7 @@ def lib():
         L.dda_op_set_rhs_members.argtypes = [vp, ctypes.POINTER(i32), i32]
         for name in EXPORTS:
             f = getattr(L, name)
-            if name not in ("dda_last_error", "dda_fft_fit"):
+            if name not in ("dda_last_error", "dda_fft_fit", "dda_fft_fit_static"):
                 f.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -186,6 +188,15 @@ def _dp(a):
 def fft_fit(x, override=0, support_non_pot=True):
     """DDA/DDAParams.cpp:45-90 (x = 2 * box extent)."""
     return int(lib().dda_fft_fit(x, override, int(support_non_pot)))
+
+
+def grid_plan(grid, precision):
+    """(static_x, static_z): whether an operator on `grid` runs the X passes and
+    the Z pass on compile-time planned kernels (host-only query)."""
+    g = (ctypes.c_uint32 * 3)(*[int(v) for v in grid])
+    sx, sz = ctypes.c_int(), ctypes.c_int()
+    _check(lib().dda_grid_plan(g, int(precision), ctypes.byref(sx), ctypes.byref(sz)), "dda_grid_plan")
+    return bool(sx.value), bool(sz.value)
 
 
 _live_contexts = weakref.WeakSet()

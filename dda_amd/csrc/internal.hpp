@@ -66,8 +66,11 @@ dda_status make_twiddles(int n, int prec, void** out, hipStream_t st);
 // The FP32 twin of an FP64 operator: same geometry, tensor converted from the
 // FP64 one (or taken from the same cache).
 dda_status create_twin32(const struct ::dda_op_s* op, struct ::dda_op_s** out);
-// Whether this thread's last tensor-cache lookup found its entry.
+// Whether this thread's last tensor-cache lookup found its entry, and the
+// seconds this thread has spent in lookups so far (the tensor's generation on
+// a miss, the wait for another caller's generation of it on a hit).
 bool last_tensor_cache_hit();
+double tensor_cache_seconds();
 
 // Launch helpers implemented in matvec.hip.
 // nrhs right-hand sides stored back to back ([nrhs][3][S]); done (nullable)

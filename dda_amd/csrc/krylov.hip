@@ -801,6 +801,15 @@ __global__ __launch_bounds__(kRedThreads) void k_norm64(const double2* __restric
 // Host side: one batched Krylov core per precision
 // ------------------------------------------------------------------------------------
 
+// Whether the process runs under rocprofv3, which hands its settings to the
+// profiled program in ROCPROF_* environment variables.
+static bool profiler_attached() {
+  static const bool attached = std::getenv("ROCPROF_OUTPUT_PATH") != nullptr ||
+                               std::getenv("ROCPROF_KERNEL_TRACE") != nullptr ||
+                               std::getenv("ROCPROF_COUNTER_COLLECTION") != nullptr;
+  return attached;
+}
+
 #define DDA_OFF(field) ((int)(offsetof(SolverState, field)))
 #define DDA_IOFF(field) ((int)(offsetof(SolverState, field) / sizeof(int)))
 
@@ -1025,8 +1034,12 @@ struct Core {
     };
 
     // The graph of two iterations: captured from the eager sequence (which
-    // leaves the vector roles as they were), then replayed.
-    const bool use_graph = !op->prof && !std::getenv("DDA_NO_GRAPHS");
+    // leaves the vector roles as they were), then replayed. Not under a
+    // profiler: a kernel-trace run of a 128-RHS block solve ended in a host
+    // segmentation fault inside the HIP runtime's graph launch (a read of a
+    // device address) while the profiler intercepted the graph's dispatches;
+    // the same graphs replay cleanly without it.
+    const bool use_graph = !op->prof && !std::getenv("DDA_NO_GRAPHS") && !profiler_attached();
     auto launch_pair = [&]() -> dda_status {
       std::vector<const void*> sig = graph_signature();
       if (!gexec || sig != gsig) {

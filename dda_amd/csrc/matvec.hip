@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <condition_variable>
 #include <map>
 #include <mutex>
@@ -999,6 +1000,19 @@ inline bool has_static_plan(int n) {
   }
 }
 
+// The static Z pass (k_pass3s) addresses A2 and each component of the tensor
+// octant with 32-bit byte offsets from uniform bases: a tensor component spans
+// (Gx/2+1)(Gy/2+1)(Gz/2+1) elements, and one buffer range of A2 at most
+// Gz/8 planes of Gy*Gx elements (its plans have at least 8 points per
+// thread). Grids beyond that run the run-time planned Z pass, whose addressing
+// is 64-bit.
+inline bool static_z_fits(int Gx, int Gy, int Gz, size_t csz) {
+  const uint64_t lim = 1ull << 32;
+  const uint64_t oct = (uint64_t)(Gx / 2 + 1) * (Gy / 2 + 1) * (Gz / 2 + 1);
+  const uint64_t span = (uint64_t)(Gz / 8 + 1) * Gy * Gx;
+  return has_static_plan(Gz) && oct * csz < lim && span * csz < lim;
+}
+
 template <typename R, bool INV>
 dda_status launch_col_any(const void* in, void* out, const ColParams& cp, const FftPlan& p, int T, const void* tw,
                           int nbatch, hipStream_t st, RhsFlags done, int wide = -1) {
@@ -1057,10 +1071,14 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     set_error("matvec: number of right-hand sides exceeds the operator's capacity");
     return DDA_INVALID_ARG;
   }
-  if (3ll * op->dims.Bz * nrhs > 65535) {
-    set_error("matvec: 3 * box z * right-hand sides exceeds the 65535 column batches of one launch");
+  if (nrhs > 65535 || 3 * op->dims.Bz > 65535) {
+    set_error("matvec: more than 65535 right-hand sides, or 3 * box z above 65535");
     return DDA_NOT_SUPPORTED;
   }
+  // The Y passes launch one grid row per (RHS, component, z) column batch; a
+  // launch holds at most 65535 rows, so larger blocks run as several launches
+  // of whole right-hand sides.
+  const int rhs_per_col_launch = 65535 / (3 * op->dims.Bz);
   RhsFlags done;
   done.done = done_ptr;
   done.stride = done_stride;
@@ -1118,8 +1136,14 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.By * d.Gx;
     cp.bstride_out = (long long)d.Gy * d.Gx;
-    DDA_TRY((launch_col_any<R, false>(a1, a2, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done,
-                                      op->tune.col_wide[0])));
+    for (int r0 = 0; r0 < nrhs; r0 += rhs_per_col_launch) {
+      const int k = std::min(rhs_per_col_launch, nrhs - r0);
+      const long long nb0 = 3ll * d.Bz * r0;
+      RhsFlags f = col_done;
+      if (f.done) f.done += (long long)r0 * f.stride;
+      DDA_TRY((launch_col_any<R, false>(a1 + nb0 * cp.bstride_in, a2 + nb0 * cp.bstride_out, cp, op->plan[1],
+                                        op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[0])));
+    }
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
   if (op->static_z) {
@@ -1171,8 +1195,14 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.Gy * d.Gx;
     cp.bstride_out = (long long)d.By * d.Gx;
-    DDA_TRY((launch_col_any<R, true>(a2, a1, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st, col_done,
-                                     op->tune.col_wide[1])));
+    for (int r0 = 0; r0 < nrhs; r0 += rhs_per_col_launch) {
+      const int k = std::min(rhs_per_col_launch, nrhs - r0);
+      const long long nb0 = 3ll * d.Bz * r0;
+      RhsFlags f = col_done;
+      if (f.done) f.done += (long long)r0 * f.stride;
+      DDA_TRY((launch_col_any<R, true>(a2 + nb0 * cp.bstride_in, a1 + nb0 * cp.bstride_out, cp, op->plan[1],
+                                       op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[1])));
+    }
   }
   // pass 5
   if (op->static_x) {
@@ -1323,7 +1353,9 @@ static void cache_evict(dda_dhat_cache_s* c, size_t need) {
 // cached, else from the Green's function clipped at G/2) while other callers of
 // the same key wait.
 static thread_local bool t_cache_hit = false;
+static thread_local double t_cache_seconds = 0;
 bool dda::last_tensor_cache_hit() { return t_cache_hit; }
+double dda::tensor_cache_seconds() { return t_cache_seconds; }
 
 static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
   const MvDims& d = op->dims;
@@ -1331,6 +1363,8 @@ static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
   const auto key = cache_key(d, kd, op->prec);
   const size_t bytes = 6ull * (d.Gx / 2 + 1) * (d.Gy / 2 + 1) * (d.Gz / 2 + 1) *
                        (op->prec == 64 ? sizeof(double2) : sizeof(float2));
+  const auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
   std::unique_lock<std::mutex> lk(c->mu);
   auto it = c->map.find(key);
   while (it != c->map.end() && !it->second.ready) {
@@ -1343,12 +1377,15 @@ static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
     it->second.last_use = ++c->clock;
     c->hits++;
     *out = it->second.ptr;
+    t_cache_seconds += elapsed();  // waiting for another caller's build counts
     return DDA_OK;
   }
   c->misses++;
   cache_evict(c, bytes);
-  dda_dhat_cache_s::Entry& en = c->map[key];
-  en.bytes = bytes;
+  // the entry's bytes are reserved now, so that concurrent misses see them
+  // when they evict; the pointer is published under the lock once built
+  c->bytes += bytes;
+  c->map[key].bytes = bytes;
   void* src64 = nullptr;
   if (op->prec == 32) {
     auto f = c->map.find(cache_key(d, kd, 64));
@@ -1359,12 +1396,14 @@ static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
   }
   lk.unlock();
   dda_status rc = DDA_OK;
-  hipError_t e = hipMalloc(&en.ptr, bytes);
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, bytes);
   if (e != hipSuccess) {
     set_error(std::string("tensor cache: hipMalloc failed: ") + hipGetErrorString(e));
     rc = e == hipErrorOutOfMemory ? DDA_OUT_OF_MEMORY : DDA_HIP_ERROR;
+    ptr = nullptr;
   } else if (src64) {
-    hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, op->ctx->stream, (const double2*)src64, (float2*)en.ptr,
+    hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, op->ctx->stream, (const double2*)src64, (float2*)ptr,
                        (long long)(bytes / sizeof(float2)));
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(op->ctx->stream);
@@ -1373,22 +1412,25 @@ static dda_status cache_get(dda_dhat_cache_s* c, dda_op_s* op, void** out) {
       rc = DDA_HIP_ERROR;
     }
   } else {
-    rc = generate_dhat(op, d.Gx / 2, d.Gy / 2, d.Gz / 2, en.ptr);
+    rc = generate_dhat(op, d.Gx / 2, d.Gy / 2, d.Gz / 2, ptr);
   }
   lk.lock();
   if (src64) c->map.find(cache_key(d, kd, 64))->second.refs--;
   if (rc != DDA_OK) {
-    if (en.ptr) (void)hipFree(en.ptr);
+    if (ptr) (void)hipFree(ptr);
+    c->bytes -= bytes;
     c->map.erase(key);
     c->cv.notify_all();
     return rc;
   }
+  dda_dhat_cache_s::Entry& en = c->map[key];
+  en.ptr = ptr;
   en.ready = true;
   en.refs = 1;
   en.last_use = ++c->clock;
-  c->bytes += bytes;
   c->cv.notify_all();
-  *out = en.ptr;
+  *out = ptr;
+  t_cache_seconds += elapsed();
   return DDA_OK;
 }
 
@@ -1448,6 +1490,17 @@ extern "C" dda_status dda_dhat_cache_stats(dda_dhat_cache_t c, uint64_t counts[3
   }
   if (bytes) *bytes = c->bytes;
   if (entries) *entries = (int)c->map.size();
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_grid_plan(const uint32_t grid[3], int precision, int* static_x, int* static_z) {
+  if (!grid || (precision != 32 && precision != 64)) {
+    set_error("dda_grid_plan: null grid or precision not 32/64");
+    return DDA_INVALID_ARG;
+  }
+  const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
+  if (static_x) *static_x = has_static_plan((int)grid[0]) ? 1 : 0;
+  if (static_z) *static_z = static_z_fits((int)grid[0], (int)grid[1], (int)grid[2], csz) ? 1 : 0;
   return DDA_OK;
 }
 
@@ -1599,12 +1652,12 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
   op->n_nonempty = (int)row_list.size();
   d.sorted = 1;  // the reference's (z, y, x) dipole order makes the row index the identity
   for (uint32_t k = 0; k < ds.nv && d.sorted; k++) d.sorted = row_dip[k] == k;
-  op->static_z = has_static_plan(d.Gz) && !std::getenv("DDA_GENERIC_FFT");
+  const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
+  op->static_z = static_z_fits(d.Gx, d.Gy, d.Gz, csz) && !std::getenv("DDA_GENERIC_FFT");
   op->static_x = has_static_plan(d.Gx) && !std::getenv("DDA_GENERIC_FFT");
 
   // Launch shapes: X passes hold ~2048 points per work-group; column passes
   // take 128-byte row segments (8 complex doubles, 16 complex floats).
-  const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
   op->p1_rows = std::max(1, 2048 / (3 * d.Gx));
   op->col_t = precision == 64 ? 8 : 16;
   while (op->col_t > 1 && std::max(d.Gy, d.Gz) * op->col_t > 16 * 256) op->col_t /= 2;
@@ -2289,11 +2342,11 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
         cp.bstride_out = (long long)(inv ? d.By : d.Gy) * d.Gx;
         const void* in = inv ? op->a2 : op->a1;
         void* out = inv ? op->a1 : op->a2;
+        // one launch's worth of column batches (a matvec splits larger blocks)
+        const int nb = 3 * d.Bz * std::min(nrhs, 65535 / (3 * d.Bz));
         rc = time_pass(st, reps, [&]() {
-          return inv ? launch_col_any<R, true>(in, out, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st,
-                                               RhsFlags(), w)
-                     : launch_col_any<R, false>(in, out, cp, op->plan[1], op->col_t, op->tw[1], 3 * d.Bz * nrhs, st,
-                                                RhsFlags(), w);
+          return inv ? launch_col_any<R, true>(in, out, cp, op->plan[1], op->col_t, op->tw[1], nb, st, RhsFlags(), w)
+                     : launch_col_any<R, false>(in, out, cp, op->plan[1], op->col_t, op->tw[1], nb, st, RhsFlags(), w);
         }, &t[w]);
       }
       if (rc != DDA_OK) {

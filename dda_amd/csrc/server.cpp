@@ -26,6 +26,7 @@
 namespace dda {
 void set_error(const std::string& msg);  // api.hip: the message dda_last_error returns
 bool last_tensor_cache_hit();            // matvec.hip: this thread's last cached-tensor lookup hit
+double tensor_cache_seconds();           // matvec.hip: this thread's seconds in tensor-cache lookups
 }
 
 namespace {
@@ -151,7 +152,7 @@ void dda_server_s::run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::
   for (Job* j : pack) k += j->nrhs;
   dda_op_t op = nullptr;
   dda_solver_t s = nullptr;
-  const double t0 = now();
+  const double t0 = now(), tc0 = dda::tensor_cache_seconds();
   dda_status rc = dda_op_create_pack(ctx, descs.data(), nm, 64, cache, &op);
   const bool hit = rc == DDA_OK && dda::last_tensor_cache_hit();
   int nmember = 0;
@@ -193,6 +194,7 @@ void dda_server_s::run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::
   if (rc == DDA_OK) rc = dda_op_set_rhs_members(op, member.data(), k);
   if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
   const double setup = now() - t0;
+  const double tensor_s = dda::tensor_cache_seconds() - tc0;  // FP64 tensor and (mixed) its FP32 twin
   if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
   const bool call_failed = rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED ||
                            (rc == DDA_INVALID_ARG && !s);
@@ -221,6 +223,7 @@ void dda_server_s::run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::
     j->res.packed_rhs = k;
     j->res.cache_hit = hit ? 1 : 0;
     j->res.setup_s = setup;
+    j->res.tensor_s = tensor_s;
     j->res.iterations = 0;
     for (int r = 0; r < j->nrhs; r++) j->res.iterations = std::max<int>(j->res.iterations, (int)info[r0 + r].iterations);
     j->error = err;
@@ -260,12 +263,13 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
   d.wave_num = j0.wave_num;
   dda_op_t op = nullptr;
   dda_solver_t s = nullptr;
-  const double t0 = now();
+  const double t0 = now(), tc0 = dda::tensor_cache_seconds();
   dda_status rc = dda_op_create_cached(ctx, &d, 64, cache, &op);
   const bool hit = rc == DDA_OK && dda::last_tensor_cache_hit();
   if (rc == DDA_OK) rc = dda_op_set_cc_sqrt_batch(op, cc.data(), k);
   if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
   const double setup = now() - t0;
+  const double tensor_s = dda::tensor_cache_seconds() - tc0;  // FP64 tensor and (mixed) its FP32 twin
   if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
   const bool call_failed = rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED ||
                            (rc == DDA_INVALID_ARG && !s);
@@ -283,6 +287,7 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
     j->res.packed_rhs = k;
     j->res.cache_hit = hit ? 1 : 0;
     j->res.setup_s = setup;
+    j->res.tensor_s = tensor_s;
     j->res.iterations = 0;
     for (int r = 0; r < j->nrhs; r++) j->res.iterations = std::max<int>(j->res.iterations, (int)info[r0 + r].iterations);
     j->error = err;

@@ -83,6 +83,15 @@ uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_pot);
  * the reference's GPU path likewise rounds to powers of two. */
 uint32_t dda_fft_fit_static(uint32_t x);
 
+/* Which passes of an operator on `grid` run compile-time planned kernels (no
+ * counterpart in the reference, whose FFT plans are chosen at run time,
+ * DDA/GpuFFTPlans.cpp:34-61): static_x for the X passes (Gx has a static
+ * plan), static_z for the Z pass with the tensor product (Gz has one and every
+ * byte offset of that pass fits 32 bits: the tensor octant's components and
+ * Gz/8 planes of the grid each below 4 GiB). Other grids use the run-time
+ * planned kernels. Host-only. */
+dda_status dda_grid_plan(const uint32_t grid[3], int precision, int* static_x, int* static_z);
+
 /* Operator description: the parts of DDAParams/DipoleGeometry the matvec reads
  * (DDA/GpuMatVec.cpp:88-136). */
 typedef struct dda_op_desc {
@@ -349,6 +358,8 @@ typedef struct dda_job_result {
   int32_t cache_hit;               /* tensor found in the cache */
   int32_t iterations;              /* most Krylov iterations among the job's fields */
   double setup_s;                  /* operator, tensor and solver set-up of the block solve */
+  double tensor_s;                 /* of setup_s: interaction tensors from the cache (their generation
+                                      on a miss, the wait for another slot's generation on a hit) */
 } dda_job_result;
 dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out);
 /* Copies the job's inputs and returns its id at once. */

@@ -13,10 +13,29 @@ from dda_amd import problem as P
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+def pinwheel_positions():
+    """A chiral dipole set: nine dipoles of a twisted arm, repeated by quarter
+    turns about the z axis through the centre of an 8x8x3 box, sorted by
+    (z, y, x). It is symmetric under those quarter turns and under no mirror
+    plane."""
+    B = 8
+    arm = [(4, 3, 0), (5, 3, 0), (6, 3, 0), (7, 3, 0), (7, 4, 1), (7, 5, 1), (6, 6, 2), (4, 4, 0), (5, 4, 1)]
+    pts = set()
+    for (x, y, z) in arm:
+        for _ in range(4):
+            pts.add((x, y, z))
+            x, y = B - 1 - y, x
+    return np.array(sorted(pts, key=lambda p: (p[2], p[1], p[0])), np.uint32)
+
+
 def _geom(name):
     d = 0.5e-6
+    if name == "pinwheel":  # chiral, quarter-turn symmetric about z (32 dipoles)
+        return P.DipoleGeometry(pinwheel_positions(), None, [1.5 + 0.1j], d), 3e-6
     if name == "sphere_r4":
         return P.sphere(4.0, 1.5 + 0.1j, d), 6e-6
+    if name == "sphere_m3_r8":  # high index: erratic residuals (stagnation tests)
+        return P.sphere(8.0, 3.0 + 0.01j, d), 6e-6
     if name == "sphere_r7":
         return P.sphere(7.0, 1.5 + 0.1j, d), 6e-6
     if name == "box_10_8_6":

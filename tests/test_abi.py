@@ -95,3 +95,18 @@ def test_load_dipole_file_fixture():
     path = os.path.join(ROOT, "tests", "golden", "box_10_8_6.geom")
     g = P.load_dipole_file(open(path).read())
     assert g.nv == 480 and g.box == (10, 8, 6)
+
+
+def test_static_z_pass_only_where_its_offsets_fit_32_bits():
+    """The static Z pass addresses the tensor octant and Gz/8 planes of the
+    grid with 32-bit byte offsets; grids beyond that fall back to the run-time
+    planned pass (64-bit addressing) instead of wrapping."""
+    assert D.grid_plan((256, 256, 256), 32) == (True, True)
+    assert D.grid_plan((256, 256, 256), 64) == (True, True)
+    # 1024^3 in FP64: octant 513^3 * 16 B = 2.16 GB, 129 planes * 16 MiB = 2.16 GB
+    assert D.grid_plan((1024, 1024, 1024), 64) == (True, True)
+    # box 672 x 784 x 512: octant 673 * 785 * 513 * 16 B = 4.34 GB > 4 GiB in FP64
+    assert D.grid_plan((1344, 1568, 1024), 64) == (False, False)
+    assert D.grid_plan((1344, 1568, 1024), 32) == (False, True)
+    # no static plan for 80 (C1's grid): run-time planned either way
+    assert D.grid_plan((80, 80, 80), 64) == (False, False)

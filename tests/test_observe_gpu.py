@@ -75,3 +75,21 @@ def test_far_field_and_jones_match_oracle(ctx):
     Js = D.jones(op, p_dev, 1, *args)
     Jsr = orc.jones(p[0], None, 1, *args)
     assert np.max(np.abs(Js - Jsr)) <= 1e-12 * np.max(np.abs(Jsr))
+
+
+def test_symmetric_jones_on_chiral_target(ctx):
+    """FarFieldCalc.cpp:163-168 on the device: for the chiral pinwheel (symmetric
+    under the quarter turn about z that takes polarisation 1 to 2, and under no
+    mirror plane), the Jones matrices from the polarisation-1 solve alone
+    (symmetric = 1) equal those from both solves to the solver's tolerance, and
+    equal the oracle's symmetric evaluation of the same solution."""
+    g, params, op, s, einc, ccs, props, pols, e_dev, p_dev = _solved(ctx, "pinwheel")
+    args = (props[0], pols[0], pols[1], 13, 0.0, math.pi / 12, 12, 0.0, math.pi / 6)
+    J2 = D.jones(op, p_dev, 0, *args)
+    J1 = D.jones(op, p_dev, 1, *args)
+    scale = np.max(np.abs(J2))
+    assert np.max(np.abs(J1 - J2)) <= 1e-8 * scale
+    p = p_dev.download().reshape(len(ccs), -1)
+    orc = cases.oracle_problem(g, params, ccs[0])
+    Jr = orc.jones(p[0], None, 1, *args)
+    assert np.max(np.abs(J1 - Jr)) <= 1e-12 * np.max(np.abs(Jr))

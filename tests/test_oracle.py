@@ -405,8 +405,8 @@ _LDR_B = (1.8915316, -0.1648469, 1.7700004)
 
 
 def _sphere_cabs_error(nd, m, mkd, prop_dir, planted=None, kind="ldr"):
-    """Mean relative error of Cabs (both polarisations) of the lattice sphere nd
-    dipoles across against Mie theory for the volume-equivalent sphere."""
+    """Mean relative error of Cabs (both polarisations) of the lattice sphere of
+    radius nd dipoles against Mie theory for the volume-equivalent sphere."""
     d = 1.0
     kd = mkd / abs(m)
     k = kd / d
@@ -466,17 +466,8 @@ def test_ldr_avgpol_is_the_polarisation_mean():
 
 
 def pinwheel_target():
-    """A chiral dipole set: nine dipoles of a twisted arm, repeated by quarter
-    turns about the z axis through the centre of an 8x8x3 box. It is
-    symmetric under those quarter turns and under no mirror plane."""
-    B = 8
-    arm = [(4, 3, 0), (5, 3, 0), (6, 3, 0), (7, 3, 0), (7, 4, 1), (7, 5, 1), (6, 6, 2), (4, 4, 0), (5, 4, 1)]
-    pts = set()
-    for (x, y, z) in arm:
-        for _ in range(4):
-            pts.add((x, y, z))
-            x, y = B - 1 - y, x
-    pos = np.array(sorted(pts, key=lambda p: (p[2], p[1], p[0])), np.uint32)
+    import cases
+    pos = cases.pinwheel_positions()
     return pos, pos.max(axis=0) + 1
 
 

@@ -112,3 +112,37 @@ def test_graph_replay_matches_eager(ctx, kind, mixed, monkeypatch):
     assert np.array_equal(pb[0], p_eager) and np.array_equal(pb[2], p_eager)
     op.set_couple_constants(cs)
     assert np.array_equal(s.get_pol_vec(einc, 1e-9), p_eager)
+
+
+def test_breakdown_status_matches_oracle(ctx):
+    """A right-hand side with b^T b = 0 started from x0 = 0: BiCG-CS stops with
+    BREAKDOWN after 0 iterations (BicgCs.cpp:68), on the device as in the
+    oracle, and returns a zero polarisation."""
+    g, params, cc, cs, prop, pol = cases.build("sphere_r4")
+    e = np.zeros(params.vec_size, complex)
+    e[0], e[1] = 1.0, 1.0j
+    orc = cases.oracle_problem(g, params, cs)
+    st, p_ref, it_ref, _ = orc.solve("bicg", e, 1e-8, max_iter=100, start=np.zeros_like(e))
+    assert st == D.BREAKDOWN and it_ref == 0
+    s = D.GpuBicgCs(_op(ctx, g, params, cs), max_iter=100)
+    p, infos = s.get_pol_vecs(e[None], 1e-8, start=np.zeros((1, params.vec_size), complex))
+    assert infos[0]["status"] == D.BREAKDOWN and infos[0]["iterations"] == 0
+    assert np.all(p == 0)
+
+
+@pytest.mark.parametrize("kind,mri", [("bicg", 1), ("bicg", 2), ("qmr", 1)])
+def test_stagnation_status_matches_oracle(ctx, kind, mri):
+    """"Too many iterations w/o increase" (IterativeSolverBase.cpp:59-60) with a
+    small max_res_increase on a high-index sphere whose residuals are erratic:
+    the device stops with STAGNATION at the oracle's iteration."""
+    print("stagnation", kind, mri, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("sphere_m3_r8")
+    einc = P.einc_plane(params, prop, pol)
+    orc = cases.oracle_problem(g, params, cs)
+    st, p_ref, it_ref, hist_ref = orc.solve(kind, einc, 1e-10, max_iter=3000, max_res_increase=mri)
+    assert st == D.STAGNATION
+    s = D.SOLVER_CLASSES[kind](_op(ctx, g, params, cs), max_iter=3000, max_res_increase=mri)
+    p, infos = s.get_pol_vecs(einc[None], 1e-10)
+    assert infos[0]["status"] == D.STAGNATION and infos[0]["iterations"] == it_ref
+    np.testing.assert_allclose(s.history(), hist_ref, rtol=1e-6)
+    assert cases.rel_err(p[0], p_ref) <= 1e-6
