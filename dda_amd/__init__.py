@@ -26,7 +26,8 @@ OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, MAX_ITER, STAGNATION, NOT_
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP_ERROR", 4: "BREAKDOWN",
                 5: "MAX_ITER", 6: "STAGNATION", 7: "NOT_SUPPORTED"}
 QMR_CS, BICGSTAB, CGNR, BICG_CS = 0, 1, 2, 3
-MODE_FP64, MODE_MIXED = 0, 1
+MODE_FP64, MODE_MIXED, MODE_FP32 = 0, 1, 2
+MODES = {"fp64": MODE_FP64, "mixed": MODE_MIXED, "fp32": MODE_FP32}
 SOLVER_KINDS = {"qmr": QMR_CS, "bicgstab": BICGSTAB, "cgnr": CGNR, "bicg": BICG_CS}
 
 # Every symbol include/dda_hip.h declares.
@@ -39,7 +40,7 @@ EXPORTS = [
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
-    "dda_solver_set_inner_floor", "dda_solver_profile_read",
+    "dda_solver_set_inner_floor", "dda_solver_profile_read", "dda_solver_profiling_run",
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
     "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_incident_plane", "dda_cross_sections",
@@ -144,6 +145,7 @@ def lib():
         L.dda_solver_set_inner_floor.argtypes = [vp, dbl]
         L.dda_solver_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_destroy.argtypes = [vp]
+        L.dda_solver_profiling_run.argtypes = [vp, dp, dp, dp]
         L.dda_solver_solve.argtypes = [vp, dp, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_solve_batch.argtypes = [vp, dp, i32, dbl, dp, dp, ctypes.POINTER(SolveInfo)]
         L.dda_solver_history.argtypes = [vp, dp, u64, ctypes.POINTER(u64)]
@@ -518,20 +520,25 @@ class MatVecGpu:
 
 
 class IterativeSolver:
-    """IterativeSolverBase<T> over an FP64 GpuMatVec: getPolVec(einc, eps, start).
+    """IterativeSolverBase<T> over a GpuMatVec: getPolVec(einc, eps, start).
 
-    mixed=True runs the Krylov iteration in FP32 inside FP64 iterative
+    mode "fp64" (default) runs everything in FP64 on an FP64 operator;
+    "fp32" is the reference's float path (GpuQmrCs<float> over GpuMatVec<float>):
+    the operator may be precision 32 or 64 (an FP32 twin is built); "mixed"
+    (or mixed=True) runs the Krylov iteration in FP32 inside FP64 iterative
     refinement (no counterpart in the reference); `inner_floor` is the
     tolerance floor of the FP32 inner solves."""
 
     kind = QMR_CS
 
-    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0, mixed=False, inner_floor=None):
+    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0, mixed=False, inner_floor=None, mode=None):
         self.op = gpu_matvec
-        self.mixed = bool(mixed)
+        mode = mode if mode is not None else ("mixed" if mixed else "fp64")
+        self.mode = mode
+        self.mixed = mode == "mixed"
         h = ctypes.c_void_p()
         _check(lib().dda_solver_create_ex(gpu_matvec.handle, self.kind, int(max_iter), int(max_res_increase),
-                                          MODE_MIXED if mixed else MODE_FP64, ctypes.byref(h)), "dda_solver_create")
+                                          MODES[mode], ctypes.byref(h)), "dda_solver_create")
         if inner_floor is not None:
             _check(lib().dda_solver_set_inner_floor(h, float(inner_floor)))
         self.handle = h
@@ -598,6 +605,16 @@ class IterativeSolver:
         sk = ctypes.c_uint64()
         _check(lib().dda_solver_profile_read(self.handle, _dp(ms), cnt, ctypes.byref(sk)))
         return ms, np.array(list(cnt), np.int64), int(sk.value)
+
+    def profiling_run(self):
+        """IterativeSolverBase::profilingRun (IterativeSolverBase.cpp:105-136):
+        five iterations timed one by one -> (microseconds in run order, median,
+        spread)."""
+        us = np.zeros(5)
+        med, spread = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dda_solver_profiling_run(self.handle, _dp(us), ctypes.byref(med), ctypes.byref(spread)),
+               "profilingRun")
+        return us, med.value, spread.value
 
     def history(self):
         n = int(self.info["iterations"]) + 1 if self.info else 1

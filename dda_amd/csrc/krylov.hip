@@ -786,6 +786,20 @@ __global__ __launch_bounds__(kRedThreads) void k_resid64(double2* r, const doubl
   }
   write_partials(part, 3, s, tr, ti);
 }
+// |b - t|^2 per right-hand side of vectors of any precision, summed in FP64.
+template <typename C>
+__global__ __launch_bounds__(kRedThreads) void k_resid_norm(const C* __restrict__ b, const C* __restrict__ t,
+                                                            long long n, double* part, const int* mask) {
+  if (mask[blockIdx.y]) return;
+  b += blockIdx.y * n;
+  t += blockIdx.y * n;
+  double s = 0;
+  DDA_GRID_LOOP(i) {
+    const double x = (double)b[i].x - (double)t[i].x, y = (double)b[i].y - (double)t[i].y;
+    s += x * x + y * y;
+  }
+  write_partials(part, 1, s, 0, 0);
+}
 __global__ __launch_bounds__(kRedThreads) void k_norm64(const double2* __restrict__ a, long long n, double* part) {
   a += blockIdx.y * n;
   double s = 0, tr = 0, ti = 0;
@@ -915,8 +929,12 @@ struct Core {
   // IterativeSolverBase.cpp:40-103) for right-hand sides 0..k-1 with eps[r];
   // active[r] == 0 leaves r untouched (done from the start). Leaves x in v[2]
   // and the final states in fin[0..k-1].
+  // profile_us (nullable): instead of iterating to convergence, run one
+  // untimed iteration and then profile_n iterations, each enqueued eagerly and
+  // waited for, and write their wall times (microseconds) there
+  // (IterativeSolverBase::profilingRun, IterativeSolverBase.cpp:105-136).
   dda_status solve(int k, const double* eps, const int* active, bool have_x0, SolverState* fin,
-                   const uint64_t* max_iters = nullptr) {
+                   const uint64_t* max_iters = nullptr, double* profile_us = nullptr, int profile_n = 0) {
     nrhs = k;
     C* bv = v[0];
     C* r = v[1];
@@ -1065,6 +1083,19 @@ struct Core {
       return DDA_OK;
     };
 
+    if (profile_us) {
+      for (int i = 0; i <= profile_n; i++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        DDA_TRY(enqueue_iteration());
+        DDA_TRY_HIP(hipStreamSynchronize(st));
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        if (i > 0) profile_us[i - 1] = us;
+      }
+      DDA_TRY_HIP(hipMemcpyAsync(fin, dst, k * sizeof(SolverState), hipMemcpyDeviceToHost, st));
+      DDA_TRY_HIP(hipStreamSynchronize(st));
+      return DDA_OK;
+    }
+
     // Chunks of iterations are enqueued ahead of the state reads: chunk c+1 is
     // queued before the states copied after chunk c are examined, so the GPU
     // never waits on the host; the chunk length follows the slowest observed
@@ -1154,15 +1185,15 @@ template struct Core<double>;
 using namespace dda;
 
 struct dda_solver_s {
-  dda_op_s* op = nullptr;        // FP64 operator (caller's)
-  dda_op_s* op32 = nullptr;      // FP32 twin (mixed mode), owned
+  dda_op_s* op = nullptr;        // caller's operator (FP64; FP32 mode also takes an FP32 one)
+  dda_op_s* op32 = nullptr;      // FP32 twin of an FP64 operator (mixed and FP32 modes), owned
   int kind = 0;
   int mode = DDA_MODE_FP64;
   uint64_t max_iter = 0, max_res_increase = 0;
   long long n = 0;
   int cap = 0;                   // right-hand sides allocated
   Core<double> c64;              // FP64 mode
-  Core<float> c32;               // mixed mode inner solver
+  Core<float> c32;               // mixed mode inner solver; FP32 mode
   double2* x64 = nullptr;        // [cap][n] FP64 vectors
   double2* b64 = nullptr;
   double2* r64 = nullptr;
@@ -1177,6 +1208,9 @@ struct dda_solver_s {
 };
 
 namespace {
+
+// The FP32 operator the iteration of the mixed and FP32 modes runs on.
+dda_op_s* op_fp32(const dda_solver_s* s) { return s->op32 ? s->op32 : s->op; }
 
 // Host sums of the per-RHS partials: quantity 0 into nrm; with rho, also
 // |sum q1 + i q2| / q0 (|r^T r| / |r|^2 after k_resid64 / k_norm64).
@@ -1210,8 +1244,10 @@ dda_status alloc_buffers(dda_solver_s* s, int k) {
   const int nb = red_blocks(s->n);
   const size_t vb = (size_t)k * s->n * sizeof(double2);
   DDA_TRY_HIP(hipMalloc(&s->io, 2 * vb));
-  DDA_TRY_HIP(hipMalloc(&s->t64, vb));
-  DDA_TRY_HIP(hipMalloc(&s->r64, vb));
+  if (s->mode != DDA_MODE_FP32) {
+    DDA_TRY_HIP(hipMalloc(&s->t64, vb));
+    DDA_TRY_HIP(hipMalloc(&s->r64, vb));
+  }
   DDA_TRY_HIP(hipMalloc(&s->part64, (size_t)k * 4 * nb * sizeof(double)));
   DDA_TRY_HIP(hipMalloc(&s->scale, k * sizeof(double)));
   DDA_TRY_HIP(hipMalloc(&s->mask, k * sizeof(int)));
@@ -1219,6 +1255,8 @@ dda_status alloc_buffers(dda_solver_s* s, int k) {
     DDA_TRY_HIP(hipMalloc(&s->x64, vb));
     DDA_TRY_HIP(hipMalloc(&s->b64, vb));
     DDA_TRY(s->c32.alloc(s->op32, s->kind, s->max_iter, s->max_res_increase, k));
+  } else if (s->mode == DDA_MODE_FP32) {
+    DDA_TRY(s->c32.alloc(op_fp32(s), s->kind, s->max_iter, s->max_res_increase, k));
   } else {
     DDA_TRY(s->c64.alloc(op, s->kind, s->max_iter, s->max_res_increase, k));
   }
@@ -1273,6 +1311,50 @@ dda_status run_fp64(dda_solver_s* s, int k, double eps, bool have_start, dda_sol
   std::vector<int> none(k, 0);
   for (int r = 0; r < k; r++) none[r] = fin[r].temp > 0 ? 0 : 1;
   DDA_TRY(residual64(s, k, K.v[2], K.v[0], none, &rr));
+  dda_status worst = DDA_OK;
+  s->last_iters = fin[0].count;
+  const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
+  for (int r = 0; r < k; r++) {
+    fill_info(info, r, fin[r].count, fin[r].count * per + 2, 0, std::sqrt(fin[r].residScale * fin[r].inprodR),
+              none[r] ? 0.0 : std::sqrt(rr[r] / fin[r].temp), fin[r].status);
+    if (fin[r].status != DDA_OK && worst == DDA_OK) worst = (dda_status)fin[r].status;
+  }
+  return worst;
+}
+
+// The reference's float path (ddaCl<float>, DDA/DDAMain.cpp:723-727): the
+// whole Krylov solve on FP32 vectors with the FP32 operator, no refinement
+// (this build keeps the scalars and reductions in FP64). The true residual is
+// recomputed with the FP32 operator, summed in FP64.
+dda_status run_fp32(dda_solver_s* s, int k, double eps, bool have_start, dda_solve_info* info) {
+  dda_op_s* op = op_fp32(s);
+  hipStream_t st = op->ctx->stream;
+  const MvDims& d = op->dims;
+  Core<float>& K = s->c32;
+  const float2* ccs = (const float2*)op->ccs;
+  const double gu = op->grid_unit;
+  const int nb = red_blocks(s->n);
+  const dim3 G(nb, k), B(kRedThreads);
+  hipLaunchKernelGGL(k_setup_b<float>, G, B, 0, st, K.v[0], s->io, ccs, op->mat_dev, d.nv, d.S, d.nmat, d.member);
+  if (have_start)
+    hipLaunchKernelGGL(k_setup_start<float>, G, B, 0, st, K.v[2], s->io + (size_t)s->cap * s->n, ccs, op->mat_dev,
+                       d.nv, d.S, d.nmat, 1.0 / (gu * gu * gu), d.member);
+  DDA_TRY_HIP(hipGetLastError());
+  std::vector<SolverState> fin(k);
+  std::vector<double> epsv(k, eps);
+  DDA_TRY(K.solve(k, epsv.data(), nullptr, have_start, fin.data()));
+  hipLaunchKernelGGL(k_get_result<float>, G, B, 0, st, s->io, K.v[2], ccs, op->mat_dev, d.nv, d.S, d.nmat,
+                     gu * gu * gu, d.member);
+  DDA_TRY_HIP(hipGetLastError());
+  std::vector<int> none(k, 0);
+  for (int r = 0; r < k; r++) none[r] = fin[r].temp > 0 ? 0 : 1;
+  DDA_TRY_HIP(hipMemcpyAsync(s->mask, none.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+  DDA_TRY(launch_matvec<float>(op, K.v[2], K.v[3], 0, st, s->mask, k, 1));
+  hipLaunchKernelGGL(k_resid_norm<float2>, G, B, 0, st, (const float2*)K.v[0], (const float2*)K.v[3], s->n,
+                     s->part64, s->mask);
+  DDA_TRY_HIP(hipGetLastError());
+  std::vector<double> rr;
+  DDA_TRY(read_sums(st, s->part64, nb, k, &rr));
   dda_status worst = DDA_OK;
   s->last_iters = fin[0].count;
   const uint64_t per = s->kind == DDA_SOLVER_QMR_CS || s->kind == DDA_SOLVER_BICG_CS ? 1 : 2;
@@ -1439,13 +1521,14 @@ static void free_solver(dda_solver_s* s) {
 
 extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
                                            int mode, dda_solver_t* out) {
-  if (!op || !out || kind < 0 || kind > 3 || (mode != DDA_MODE_FP64 && mode != DDA_MODE_MIXED)) {
+  if (!op || !out || kind < 0 || kind > 3 ||
+      (mode != DDA_MODE_FP64 && mode != DDA_MODE_MIXED && mode != DDA_MODE_FP32)) {
     set_error("dda_solver_create: null argument, unknown solver kind or mode");
     return DDA_INVALID_ARG;
   }
   DDA_TRY(bind_device(op->ctx));
-  if (op->prec != 64) {
-    set_error("dda_solver_create: the operator must be FP64 (mixed mode builds its own FP32 twin)");
+  if (op->prec != 64 && mode != DDA_MODE_FP32) {
+    set_error("dda_solver_create: FP64 and mixed modes need an FP64 operator (mixed mode builds its own FP32 twin)");
     return DDA_INVALID_ARG;
   }
   *out = nullptr;
@@ -1459,7 +1542,7 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
   s->max_res_increase = max_res_increase ? max_res_increase : defaults[kind];
   s->n = 3ll * d.S;
   dda_status rc = DDA_OK;
-  if (mode == DDA_MODE_MIXED) {
+  if (mode == DDA_MODE_MIXED || (mode == DDA_MODE_FP32 && op->prec == 64)) {
     rc = create_twin32(op, &s->op32);
     if (rc == DDA_OK && op->tuned) rc = dda_op_tune(s->op32, 1, nullptr, 0);
   }
@@ -1518,7 +1601,7 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = op->ctx->stream;
   DDA_TRY(alloc_buffers(s, nrhs));
-  if (s->mode == DDA_MODE_MIXED) {
+  if (s->op32) {  // the FP32 twin of the mixed and FP32 modes follows the caller's operator
     DDA_TRY(dda_op_set_cc_sqrt_batch(s->op32, op->cc_sqrt_host.data(), nrhs));
     if (op->pack.nmember) DDA_TRY(dda_op_set_rhs_members(s->op32, op->pack.member_host.data(), nrhs));
     s->op32->prof = op->prof;
@@ -1528,8 +1611,9 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
   DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, bytes, in_kind, st));
   if (start) DDA_TRY_HIP(hipMemcpyAsync(s->io + (size_t)s->cap * s->n, start, bytes, in_kind, st));
   std::vector<dda_solve_info> inf(nrhs);
-  dda_status rc = s->mode == DDA_MODE_MIXED ? run_mixed(s, nrhs, eps, start != nullptr, inf.data())
-                                            : run_fp64(s, nrhs, eps, start != nullptr, inf.data());
+  dda_status rc = s->mode == DDA_MODE_MIXED  ? run_mixed(s, nrhs, eps, start != nullptr, inf.data())
+                  : s->mode == DDA_MODE_FP32 ? run_fp32(s, nrhs, eps, start != nullptr, inf.data())
+                                             : run_fp64(s, nrhs, eps, start != nullptr, inf.data());
   if (rc == DDA_HIP_ERROR || rc == DDA_OUT_OF_MEMORY || rc == DDA_NOT_SUPPORTED) return rc;
   DDA_TRY(zero_invalid(s, nrhs, inf.data()));
   DDA_TRY_HIP(hipMemcpyAsync(pol_out, s->io, bytes, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
@@ -1571,9 +1655,12 @@ extern "C" dda_status dda_solver_history(dda_solver_t s, double* out, uint64_t n
     m = std::min<uint64_t>(n, s->outer_hist.size());
     for (uint64_t i = 0; i < m; i++) out[i] = s->outer_hist[i];
   } else {
-    m = std::min<uint64_t>(std::min<uint64_t>(n, s->last_iters + 1), s->c64.hist_cap);
+    const bool f32 = s->mode == DDA_MODE_FP32;
+    const uint64_t cap = f32 ? s->c32.hist_cap : s->c64.hist_cap;
+    const double* hist = f32 ? s->c32.hist : s->c64.hist;
+    m = std::min<uint64_t>(std::min<uint64_t>(n, s->last_iters + 1), cap);
     if (m) {
-      DDA_TRY_HIP(hipMemcpyAsync(out, s->c64.hist, m * sizeof(double), hipMemcpyDeviceToHost, s->op->ctx->stream));
+      DDA_TRY_HIP(hipMemcpyAsync(out, hist, m * sizeof(double), hipMemcpyDeviceToHost, s->op->ctx->stream));
       DDA_TRY_HIP(hipStreamSynchronize(s->op->ctx->stream));
     }
   }
@@ -1595,5 +1682,65 @@ extern "C" dda_status dda_solver_profile_read(dda_solver_t s, double ms[5], uint
     set_error("dda_solver_profile_read: null solver");
     return DDA_INVALID_ARG;
   }
-  return dda_op_profile_read(s->mode == DDA_MODE_MIXED ? s->op32 : s->op, ms, counts, skipped);
+  return dda_op_profile_read(s->mode == DDA_MODE_FP64 ? s->op : op_fp32(s), ms, counts, skipped);
+}
+
+extern "C" dda_status dda_solver_profiling_run(dda_solver_t s, double us[5], double* median_us, double* spread_us) {
+  if (!s || !us) {
+    set_error("dda_solver_profiling_run: null argument");
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY(bind_device(s->op->ctx));
+  dda_op_s* op = s->op;
+  if (!op->cc_set) {
+    set_error("dda_solver_profiling_run: couple constants have not been set");
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY(alloc_buffers(s, 1));
+  if (s->op32) DDA_TRY(dda_op_set_cc_sqrt_batch(s->op32, op->cc_sqrt_host.data(), 1));
+  // a fixed pseudo-random field (re, im uniform in [-1, 1) on every dipole):
+  // the reference times iterations on whatever its vectors hold
+  const MvDims& d = op->dims;
+  std::vector<double2> e((size_t)s->n, make_double2(0, 0));
+  uint64_t x = 0x5EEDull;
+  auto next = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  };
+  for (int c = 0; c < 3; c++)
+    for (int i = 0; i < d.nv; i++) e[(size_t)c * d.S + i] = make_double2(next(), next());
+  hipStream_t st = op->ctx->stream;
+  DDA_TRY_HIP(hipMemcpyAsync(s->io, e.data(), e.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+  const dim3 G(red_blocks(s->n), 1), B(kRedThreads);
+  double t[5];
+  SolverState fin;
+  const double zero = 0.0;  // eps 0: the timed iterations never meet the tolerance
+  const uint64_t big = ~0ull >> 1;
+  if (s->mode == DDA_MODE_FP64) {
+    hipLaunchKernelGGL(k_setup_b<double>, G, B, 0, st, s->c64.v[0], s->io, (const double2*)op->ccs, op->mat_dev,
+                       d.nv, d.S, d.nmat, d.member);
+    DDA_TRY_HIP(hipGetLastError());
+    DDA_TRY(s->c64.solve(1, &zero, nullptr, false, &fin, &big, t, 5));
+  } else {
+    dda_op_s* o = op_fp32(s);
+    hipLaunchKernelGGL(k_setup_b<float>, G, B, 0, st, s->c32.v[0], s->io, (const float2*)o->ccs, o->mat_dev, d.nv,
+                       d.S, d.nmat, o->dims.member);
+    DDA_TRY_HIP(hipGetLastError());
+    DDA_TRY(s->c32.solve(1, &zero, nullptr, false, &fin, &big, t, 5));
+  }
+  if (fin.done && fin.status != DDA_OK) {
+    set_error("dda_solver_profiling_run: the iteration stopped early (breakdown)");
+    return (dda_status)fin.status;
+  }
+  for (int i = 0; i < 5; i++) us[i] = t[i];
+  // median and the larger distance from it to the extremes, as the reference
+  // reports them (IterativeSolverBase.cpp:126-134)
+  double srt[5];
+  std::copy(t, t + 5, srt);
+  std::sort(srt, srt + 5);
+  if (median_us) *median_us = srt[2];
+  if (spread_us) *spread_us = std::max(srt[2] - srt[0], srt[4] - srt[2]);
+  return DDA_OK;
 }

@@ -4,14 +4,15 @@
 //
 //   Hip::GpuMatVec<T>         GpuMatVec<T>             DDA/GpuMatVec.hpp:85-99
 //   Hip::MatVecHip<T>         MatVec<T> / MatVecGpu<T> DDA/MatVec.hpp:33-55, DDA/MatVecGpu.cpp:51-68
-//   Hip::IterativeSolverBase  IterativeSolverBase<T>   DDA/IterativeSolverBase.hpp:59-62
-//   Hip::GpuQmrCs, GpuBicgStab, GpuCgnr, GpuBicgCs     DDA/GpuQmrCs.hpp:81 etc.
+//   Hip::IterativeSolverBase<T>  IterativeSolverBase<T> DDA/IterativeSolverBase.hpp:59-62
+//   Hip::GpuQmrCs<T>, GpuBicgStab<T>, GpuCgnr<T>, GpuBicgCs<T>  DDA/GpuQmrCs.hpp:81 etc.
 //
 // Status codes become exceptions (Hip::Error), as the reference's ASSERT /
 // ABORT_MSG become Core::AssertionFailure (Core/Assert.hpp:42-64). Header-only,
 // C++17, no Boost; link with libdda_hip.so.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <complex>
 #include <cstdint>
@@ -19,6 +20,8 @@
 #include <ostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/dda_hip.h"
@@ -169,18 +172,27 @@ class GpuMatVec {
   std::vector<std::array<std::complex<double>, 3>> cc_;
 };
 
-// MatVec<T> on host vectors: apply(arg, result, conj), as MatVecGpu<T>.
+// MatVec<T> on host vectors: apply(arg, result, conj), as MatVecGpu<T>
+// (vectors of std::complex<T>; the C ABI stages them as complex double).
 template <typename T>
 class MatVecHip {
  public:
   explicit MatVecHip(GpuMatVec<T>& op) : op_(op) {}
-  void apply(const std::vector<std::complex<double>>& arg, std::vector<std::complex<double>>& result, bool conj) {
+  void apply(const std::vector<std::complex<T>>& arg, std::vector<std::complex<T>>& result, bool conj) {
     const size_t n = op_.ddaParams().vecSize();
     if (arg.size() != n) throw Error(DDA_INVALID_ARG, "MatVecHip::apply: argument size");
     result.resize(n);
-    check(dda_op_apply_host(op_.handle(), reinterpret_cast<const double*>(arg.data()),
-                            reinterpret_cast<double*>(result.data()), conj ? 1 : 0),
-          "MatVecHip::apply");
+    if constexpr (std::is_same<T, double>::value) {
+      check(dda_op_apply_host(op_.handle(), reinterpret_cast<const double*>(arg.data()),
+                              reinterpret_cast<double*>(result.data()), conj ? 1 : 0),
+            "MatVecHip::apply");
+    } else {
+      std::vector<std::complex<double>> a(arg.begin(), arg.end()), r(n);
+      check(dda_op_apply_host(op_.handle(), reinterpret_cast<const double*>(a.data()),
+                              reinterpret_cast<double*>(r.data()), conj ? 1 : 0),
+            "MatVecHip::apply");
+      for (size_t i = 0; i < n; i++) result[i] = std::complex<T>(r[i]);
+    }
   }
   void setCoupleConstants(const std::vector<std::array<std::complex<double>, 3>>& ccSqrt) {
     op_.setCoupleConstants(ccSqrt);
@@ -190,9 +202,13 @@ class MatVecHip {
   GpuMatVec<T>& op_;
 };
 
-// IterativeSolverBase<T>::getPolVec over an FP64 GpuMatVec.
+// IterativeSolverBase<T>::getPolVec over a GpuMatVec<T>: T = double runs the
+// reference's double path (or, with mixed, FP32 Krylov inside FP64
+// refinement), T = float its float path (ddaCl<float>, DDA/DDAMain.cpp:723-727).
+template <typename T>
 class IterativeSolverBase {
  public:
+  typedef std::complex<T> ctype;
   virtual ~IterativeSolverBase() { dda_solver_destroy(h_); }
   IterativeSolverBase(const IterativeSolverBase&) = delete;
   IterativeSolverBase& operator=(const IterativeSolverBase&) = delete;
@@ -204,25 +220,25 @@ class IterativeSolverBase {
   // |r| < eps |b|; throws Error with DDA_MAX_ITER / DDA_STAGNATION /
   // DDA_BREAKDOWN where the reference aborts. A residual line is written to
   // `log` as the reference does at the end of each solve.
-  std::shared_ptr<std::vector<std::complex<double>>> getPolVec(const std::vector<std::complex<double>>& einc,
-                                                               double eps, std::ostream& log,
-                                                               const std::vector<std::complex<double>>& start = {}) {
+  std::shared_ptr<std::vector<ctype>> getPolVec(const std::vector<ctype>& einc, T eps, std::ostream& log,
+                                                const std::vector<ctype>& start = {}) {
     const size_t n = op_.ddaParams().vecSize();
     if (einc.size() != n || (!start.empty() && start.size() != n))
       throw Error(DDA_INVALID_ARG, "getPolVec: vector size");
-    auto out = std::make_shared<std::vector<std::complex<double>>>(n);
-    dda_status s = dda_solver_solve(h_, reinterpret_cast<const double*>(einc.data()), eps,
-                                    start.empty() ? nullptr : reinterpret_cast<const double*>(start.data()),
-                                    reinterpret_cast<double*>(out->data()), &info_);
+    const std::vector<std::complex<double>> e = widen(einc), st = widen(start);
+    std::vector<std::complex<double>> out(n);
+    dda_status s = dda_solver_solve(h_, reinterpret_cast<const double*>(e.data()), eps,
+                                    st.empty() ? nullptr : reinterpret_cast<const double*>(st.data()),
+                                    reinterpret_cast<double*>(out.data()), &info_);
     log << "iterations = " << info_.iterations << ", |r|/|b| = " << info_.rel_residual << std::endl;
     check(s, "getPolVec");
-    return out;
+    return std::make_shared<std::vector<ctype>>(out.begin(), out.end());
   }
   // Batched getPolVec: every field of `eincs` solved in lockstep, each result
   // bitwise the one getPolVec gives it alone. Per-field outcomes are in
   // lastInfos() (status per field); throws only for errors of the call itself.
-  std::vector<std::shared_ptr<std::vector<std::complex<double>>>> getPolVecs(
-      const std::vector<std::vector<std::complex<double>>>& eincs, double eps, std::ostream& log) {
+  std::vector<std::shared_ptr<std::vector<ctype>>> getPolVecs(const std::vector<std::vector<ctype>>& eincs, T eps,
+                                                              std::ostream& log) {
     const size_t n = op_.ddaParams().vecSize();
     std::vector<std::complex<double>> flat;
     flat.reserve(n * eincs.size());
@@ -235,14 +251,27 @@ class IterativeSolverBase {
     dda_status s = dda_solver_solve_batch(h_, reinterpret_cast<const double*>(flat.data()), (int)eincs.size(), eps,
                                           nullptr, reinterpret_cast<double*>(out.data()), infos_.data());
     if (s == DDA_HIP_ERROR || s == DDA_OUT_OF_MEMORY || s == DDA_NOT_SUPPORTED) check(s, "getPolVecs");
-    std::vector<std::shared_ptr<std::vector<std::complex<double>>>> res;
+    std::vector<std::shared_ptr<std::vector<ctype>>> res;
     for (size_t r = 0; r < eincs.size(); r++) {
-      res.push_back(std::make_shared<std::vector<std::complex<double>>>(out.begin() + r * n, out.begin() + (r + 1) * n));
+      res.push_back(std::make_shared<std::vector<ctype>>(out.begin() + r * n, out.begin() + (r + 1) * n));
       log << "rhs " << r << ": iterations = " << infos_[r].iterations << ", |r|/|b| = " << infos_[r].rel_residual
           << ", status = " << infos_[r].status << std::endl;
     }
     if (!infos_.empty()) info_ = infos_[0];
     return res;
+  }
+  // IterativeSolverBase::profilingRun (DDA/IterativeSolverBase.cpp:105-136):
+  // one untimed iteration, five timed ones; prints "[i]time " sorted by time,
+  // then "median +- spread", as the reference does.
+  void profilingRun(std::ostream& out, std::ostream& log) {
+    double us[5], med = 0, spread = 0;
+    check(dda_solver_profiling_run(h_, us, &med, &spread), "profilingRun");
+    std::array<std::pair<double, int>, 5> r;
+    for (int i = 0; i < 5; i++) r[i] = {us[i], i};
+    std::sort(r.begin(), r.end());
+    for (const auto& e : r) out << "[" << e.second << "]" << e.first << "us ";
+    out << std::endl << std::endl << med << "us +- " << spread << "us" << std::endl;
+    log << "profiling run: " << med << "us +- " << spread << "us per iteration" << std::endl;
   }
   const dda_solve_info& lastInfo() const { return info_; }
   const std::vector<dda_solve_info>& lastInfos() const { return infos_; }
@@ -255,36 +284,45 @@ class IterativeSolverBase {
   }
 
  protected:
-  IterativeSolverBase(GpuMatVec<double>& op, int kind, uint64_t maxIter, bool mixed) : op_(op) {
-    check(dda_solver_create_ex(op.handle(), kind, maxIter, 0, mixed ? DDA_MODE_MIXED : DDA_MODE_FP64, &h_),
-          "solver create");
+  IterativeSolverBase(GpuMatVec<T>& op, int kind, uint64_t maxIter, bool mixed) : op_(op) {
+    if (mixed && !std::is_same<T, double>::value)
+      throw Error(DDA_INVALID_ARG, "mixed precision refines in FP64: it needs a GpuMatVec<double>");
+    const int mode = std::is_same<T, float>::value ? DDA_MODE_FP32 : mixed ? DDA_MODE_MIXED : DDA_MODE_FP64;
+    check(dda_solver_create_ex(op.handle(), kind, maxIter, 0, mode, &h_), "solver create");
   }
-  GpuMatVec<double>& op_;
+  static std::vector<std::complex<double>> widen(const std::vector<ctype>& v) {
+    return std::vector<std::complex<double>>(v.begin(), v.end());
+  }
+  GpuMatVec<T>& op_;
   dda_solver_t h_ = nullptr;
   dda_solve_info info_{};
   std::vector<dda_solve_info> infos_;
 };
 
 // new GpuQmrCs<T>(pool, queues, ddaParams, matVec, maxIter, accounting) etc.
-class GpuQmrCs : public IterativeSolverBase {
+template <typename T>
+class GpuQmrCs : public IterativeSolverBase<T> {
  public:
-  GpuQmrCs(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
-      : IterativeSolverBase(op, DDA_SOLVER_QMR_CS, maxIter, mixed) {}
+  GpuQmrCs(GpuMatVec<T>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase<T>(op, DDA_SOLVER_QMR_CS, maxIter, mixed) {}
 };
-class GpuBicgStab : public IterativeSolverBase {
+template <typename T>
+class GpuBicgStab : public IterativeSolverBase<T> {
  public:
-  GpuBicgStab(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
-      : IterativeSolverBase(op, DDA_SOLVER_BICGSTAB, maxIter, mixed) {}
+  GpuBicgStab(GpuMatVec<T>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase<T>(op, DDA_SOLVER_BICGSTAB, maxIter, mixed) {}
 };
-class GpuCgnr : public IterativeSolverBase {
+template <typename T>
+class GpuCgnr : public IterativeSolverBase<T> {
  public:
-  GpuCgnr(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
-      : IterativeSolverBase(op, DDA_SOLVER_CGNR, maxIter, mixed) {}
+  GpuCgnr(GpuMatVec<T>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase<T>(op, DDA_SOLVER_CGNR, maxIter, mixed) {}
 };
-class GpuBicgCs : public IterativeSolverBase {
+template <typename T>
+class GpuBicgCs : public IterativeSolverBase<T> {
  public:
-  GpuBicgCs(GpuMatVec<double>& op, uint64_t maxIter, bool mixed = false)
-      : IterativeSolverBase(op, DDA_SOLVER_BICG_CS, maxIter, mixed) {}
+  GpuBicgCs(GpuMatVec<T>& op, uint64_t maxIter, bool mixed = false)
+      : IterativeSolverBase<T>(op, DDA_SOLVER_BICG_CS, maxIter, mixed) {}
 };
 
 }  // namespace Hip

@@ -221,11 +221,14 @@ typedef struct dda_solve_info {
 
 typedef enum dda_solver_mode {
   DDA_MODE_FP64 = 0,  /* every operation in FP64, as the reference's double path */
-  DDA_MODE_MIXED = 1  /* FP32 Krylov solves (FP64 scalars) inside FP64 iterative refinement */
+  DDA_MODE_MIXED = 1, /* FP32 Krylov solves (FP64 scalars) inside FP64 iterative refinement */
+  DDA_MODE_FP32 = 2   /* the reference's float path (ddaCl<float>, DDA/DDAMain.cpp:723-727): FP32
+                         operator and vectors, no refinement; scalars and reductions in FP64 */
 } dda_solver_mode;
 
 /* new GpuQmrCs(...) etc. (DDA/DDAMain.cpp:647-656) in FP64 mode; op must be an
- * FP64 operator. max_iter = 0 selects 3*Bx*By*Bz (DDA/DDAMain.cpp:640-644);
+ * FP64 operator (DDA_MODE_FP32 through dda_solver_create_ex also takes an FP32
+ * one, as GpuQmrCs<float> takes GpuMatVec<float>). max_iter = 0 selects 3*Bx*By*Bz (DDA/DDAMain.cpp:640-644);
  * max_res_increase = 0 selects the solver's reference default (QMR 50000,
  * BiCGStab 30000, CGNR 10, BiCG 50000). */
 dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase,
@@ -239,6 +242,9 @@ dda_status dda_solver_create(dda_op_t op, int kind, uint64_t max_iter, uint64_t 
  * steps. */
 dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64_t max_res_increase, int mode,
                                 dda_solver_t* out);
+/* DDA_MODE_FP32 runs the Krylov iteration on the FP32 operator: op itself if it
+ * is precision 32, else an FP32 twin built like the mixed mode's; the true
+ * residual in dda_solve_info is then recomputed with that FP32 operator. */
 /* Tolerance floor of the FP32 inner solves in mixed mode (default 1e-4). */
 dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
 /* dda_op_profile_read for the operator the Krylov iteration runs on: the FP32
@@ -246,6 +252,15 @@ dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
  * else the FP64 operator itself. */
 dda_status dda_solver_profile_read(dda_solver_t s, double ms[5], uint64_t counts[5], uint64_t* skipped);
 dda_status dda_solver_destroy(dda_solver_t s);
+/* IterativeSolverBase::profilingRun, DDA/IterativeSolverBase.cpp:105-136: with
+ * the couple constants already set, one untimed iteration, then five
+ * iterations timed one by one (each waited for) on a fixed pseudo-random
+ * right-hand side; us[5] receives their wall times in microseconds in the
+ * order run, median_us / spread_us (nullable) the median and the larger
+ * distance from it to the fastest or slowest, as the reference prints them.
+ * The iteration timed is the one the solves run: FP64, or FP32 in the mixed
+ * and FP32 modes. */
+dda_status dda_solver_profiling_run(dda_solver_t s, double us[5], double* median_us, double* spread_us);
 /* IterativeSolverBase::getPolVec, DDA/IterativeSolverBase.cpp:40-103.
  * einc: host [3*vec_stride] complex double; start (nullable) is a previous
  * polarisation (--load-start-dip-pol, x0 = start / sqrt(a)); pol_out receives

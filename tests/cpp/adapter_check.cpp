@@ -1,13 +1,16 @@
 // Drives the C++ adapters (dda_amd/host/dda_hip.hpp) the way the reference's
-// ddaCl / createResOutput drive GpuMatVec and GpuQmrCs (DDA/DDAMain.cpp:550-663,
-// 306-410): build the operator, set the couple constants, apply the matvec on
-// host vectors in FP64 and FP32, then getPolVec with QMR-CS.
+// ddaCl<double> and ddaCl<float> / createResOutput drive GpuMatVec and GpuQmrCs
+// (DDA/DDAMain.cpp:550-663, 306-410, 723-727): build the operator, set the
+// couple constants, apply the matvec on host vectors in FP64 and FP32, then
+// getPolVec with QMR-CS on the double path and on the float path, and the
+// profiling run (--profiling-run, DDA/DDAMain.cpp:313-317).
 //
 // usage: adapter_check <case.bin> <out.bin>
 //   case.bin: u32 nv, nmat, box[3], grid[3]; f64 gridUnit, waveNum, eps;
 //             u32 pos[nv][3]; u8 mat[nv]; c128 ccSqrt[nmat][3];
 //             c128 x[3 S]; c128 einc[3 S]      (S = roundUp(nv, 16))
-//   out.bin:  c128 y64[3 S]; c128 y32[3 S]; c128 p[3 S]; u64 iterations
+//   out.bin:  c128 y64[3 S]; c128 y32[3 S]; c128 p[3 S]; c128 p32[3 S];
+//             u64 iterations, iterations32
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -57,10 +60,24 @@ int main(int argc, char** argv) {
     MatVecHip<float> mv32(op32);
     mv64.setCoupleConstants(cc);
     mv32.setCoupleConstants(cc);
-    std::vector<cd> y64, y32;
+    std::vector<cd> y64;
+    std::vector<std::complex<float>> y32f, xf(x.begin(), x.end());
     mv64.apply(x, y64, false);
-    mv32.apply(x, y32, false);
-    GpuQmrCs solver(op64, 10000);
+    mv32.apply(xf, y32f, false);
+    std::vector<cd> y32(y32f.begin(), y32f.end());
+    // the float path: GpuQmrCs<float> over GpuMatVec<float> (ddaCl<float>)
+    std::vector<cd> p32;
+    uint64_t it32 = 0;
+    {
+      GpuQmrCs<float> s32(op32, 10000);
+      s32.setCoupleConstants(cc);
+      std::vector<std::complex<float>> ef(einc.begin(), einc.end());
+      auto pf = s32.getPolVec(ef, 1e-5f, std::cout);
+      p32.assign(pf->begin(), pf->end());
+      it32 = s32.lastInfo().iterations;
+      s32.profilingRun(std::cout, std::cout);
+    }
+    GpuQmrCs<double> solver(op64, 10000);
     solver.setCoupleConstants(cc);
     auto pol = solver.getPolVec(einc, fd[2], std::cout);
     uint64_t it = solver.lastInfo().iterations;
@@ -81,9 +98,12 @@ int main(int argc, char** argv) {
     o.write(reinterpret_cast<const char*>(y64.data()), sizeof(cd) * y64.size());
     o.write(reinterpret_cast<const char*>(y32.data()), sizeof(cd) * y32.size());
     o.write(reinterpret_cast<const char*>(pol->data()), sizeof(cd) * pol->size());
+    o.write(reinterpret_cast<const char*>(p32.data()), sizeof(cd) * p32.size());
     o.write(reinterpret_cast<const char*>(&it), sizeof it);
+    o.write(reinterpret_cast<const char*>(&it32), sizeof it32);
+    solver.profilingRun(std::cout, std::cout);
     // error path: a zero-iteration limit aborts like the reference ("Too many iterations")
-    GpuQmrCs limited(op64, 1);
+    GpuQmrCs<double> limited(op64, 1);
     limited.setCoupleConstants(cc);
     try {
       limited.getPolVec(einc, 1e-14, std::cout);

@@ -41,9 +41,13 @@ def _scat_grid():
     return g
 
 
-@pytest.mark.parametrize("kind,mixed,limit", [("qmr", False, 1e-4), ("bicgstab", False, 1e-4),
-                                              ("bicgstab", True, 1e-3), ("qmr", True, 1e-3)])
-def test_box_fixture_against_oracle(ctx, kind, mixed, limit):
+@pytest.mark.parametrize("kind,mode,limit", [("qmr", "fp64", 1e-4), ("bicgstab", "fp64", 1e-4),
+                                             ("bicgstab", "mixed", 1e-3), ("qmr", "mixed", 1e-3),
+                                             ("qmr", "fp32", 1e-3), ("bicgstab", "fp32", 1e-3)])
+def test_box_fixture_against_oracle(ctx, kind, mode, limit):
+    """mode fp32 is the reference's float path (test-f.log, DDA/OMakefile:66-67):
+    solved to the reference's default eps 1e-5 and held to its float threshold."""
+    print("compare", kind, mode, flush=True)
     g, params, cc, cs, prop, pol = cases.build("box_10_8_6")
     _, pol1, pol2 = P.beam_pols(prop)
     ccs = np.array([P.couple_constants(params, prop, p, "ldr")[1] for p in (pol1, pol2)])
@@ -53,8 +57,9 @@ def test_box_fixture_against_oracle(ctx, kind, mixed, limit):
     e_dev = D.DeviceVector(ctx, 2 * params.vec_size, 64)
     e_dev.upload(einc.reshape(-1))
     p_dev = D.DeviceVector(ctx, 2 * params.vec_size, 64)
-    s = D.SOLVER_CLASSES[kind](op, mixed=mixed, max_iter=5000)
-    infos = s.solve_device(e_dev, p_dev, 2, 1e-8)
+    s = D.SOLVER_CLASSES[kind](op, mode=mode, max_iter=5000)
+    eps = 1e-5 if mode == "fp32" else 1e-8
+    infos = s.solve_device(e_dev, p_dev, 2, eps)
     assert all(i["status"] == D.OK for i in infos)
     gr = _scat_grid()
     (nt, t0, dt), (nph, f0, df) = gr["theta"], gr["phi"]
@@ -70,9 +75,9 @@ def test_box_fixture_against_oracle(ctx, kind, mixed, limit):
     Jr = cases.oracle_problem(g, params, ccs[0]).jones(refs[0], refs[1], 0, prop, pol1, pol2, nt, t0, dt, nph, f0, df)
     dj = O.field_diff(J, Jr)
     dp = max(O.field_diff(p[r], refs[r]) for r in range(2))
-    print(kind, "mixed" if mixed else "fp64", "FieldDiff Jones %.2e, DipPol %.2e" % (dj, dp), flush=True)
+    print(kind, mode, "FieldDiff Jones %.2e, DipPol %.2e" % (dj, dp), flush=True)
     assert dj < limit and dp < limit
-    # well inside the reference's thresholds: the solves meet eps = 1e-8
-    assert dj < 1e-6 and dp < 1e-6
+    if mode != "fp32":  # well inside the reference's thresholds: the solves meet eps = 1e-8
+        assert dj < 1e-6 and dp < 1e-6
     s.close()
     op.close()
