@@ -169,4 +169,10 @@ struct dda_op_s {
   int cap_rhs = 1;   // right-hand sides the grid buffers hold
   int cc_rhs = 0;    // right-hand sides with couple constants set
   bool static_z = false;  // pass 3 uses a compile-time planned kernel
+  // A block of right-hand sides runs as two halves on two streams (the Z pass
+  // of one beside the X and Y passes of the other); the second stream and
+  // the fork / join events are made on first use.
+  bool split_rhs = true;
+  hipStream_t split_stream = nullptr;
+  hipEvent_t split_ev[2] = {nullptr, nullptr};
 };

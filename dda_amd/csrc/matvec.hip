@@ -918,8 +918,30 @@ dda_status launch_col_static(const void* in, void* out, const ColParams& cp, con
                             : launch_col_static_w<R, N, false, INV>(in, out, cp, tw, nbatch, st, done);
 }
 
+// The buffers of right-hand sides r0 .. r0 + nrhs - 1 of an operator: what one
+// sequence of the five passes works on (the whole block, or one half of it
+// when the halves run on two streams).
+struct MvView {
+  void* a1;
+  void* a2;
+  const void* ccs;
+  MvDims d;  // member table shifted to r0
+};
+
+inline MvView view_of(const dda_op_s* op, int r0) {
+  const MvDims& d = op->dims;
+  const size_t csz = op->prec == 64 ? sizeof(double2) : sizeof(float2);
+  MvView v;
+  v.a1 = (char*)op->a1 + (size_t)r0 * 3 * d.Bz * d.By * d.Gx * csz;
+  v.a2 = (char*)op->a2 + (size_t)r0 * 3 * d.Bz * d.Gy * d.Gx * csz;
+  v.ccs = (const char*)op->ccs + (size_t)r0 * 3 * d.nmat * csz;
+  v.d = d;
+  if (d.member) v.d.member = d.member + r0;
+  return v;
+}
+
 template <typename R, int N, bool W>
-dda_status launch_pass3_static_w(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
+dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
   // FP32 1024-point columns: 16 columns (128-byte segments) in a 1024-thread
   // work-group at 128 VGPRs, the CU's whole register file (17.5 ms at 1024^3,
@@ -931,59 +953,60 @@ dda_status launch_pass3_static_w(dda_op_s* op, hipStream_t st, RhsFlags done, in
                                 (sizeof(R) == 4 && N <= 64) ? 32 : 0>();
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
-  const MvDims& d = op->dims;
+  const MvDims& d = v.d;
   const size_t lds = (size_t)N * lp * sizeof(C);
   const int ntx = (d.Gx + T - 1) / T;
   const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
   const int nper = (nwork + 7) / 8;
   auto k = k_pass3s<R, N, T, W>;
   DDA_TRY(set_lds(k, lds));
-  hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)op->a2, (const C*)op->dhat, d,
+  hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)v.a2, (const C*)op->dhat, d,
                      (const C*)op->tw[2], nwork, nper, nrhs, done);
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
 }
 
 template <typename R, int N>
-dda_status launch_pass3_static(dda_op_s* op, hipStream_t st, RhsFlags done, int nrhs) {
-  return plan_wide<R>(op->tune.p3_wide) ? launch_pass3_static_w<R, N, true>(op, st, done, nrhs)
-                                        : launch_pass3_static_w<R, N, false>(op, st, done, nrhs);
+dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
+  return plan_wide<R>(op->tune.p3_wide) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
+                                        : launch_pass3_static_w<R, N, false>(op, v, st, done, nrhs);
 }
 
 template <typename R, int N, bool W>
-dda_status launch_x_static_w(dda_op_s* op, bool inverse, const void* x, void* y, int conj, hipStream_t st,
-                             RhsFlags done, int nrhs) {
+dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
+                             hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
   constexpr int TPC = N / StaticPlan<N, W>::PPT;
   constexpr int LN = 256 / TPC;
   constexpr int lp = (LN & 1) ? LN : LN + 1;
-  const MvDims& d = op->dims;
+  const MvDims& d = v.d;
   const size_t lds = (size_t)N * lp * sizeof(C);
   if (!inverse) {
     auto k = k_pass1s<R, N, W>;
     DDA_TRY(set_lds(k, lds));
-    hipLaunchKernelGGL(k, dim3((3 * d.nrows + LN - 1) / LN, nrhs), dim3(LN * TPC), lds, st, (const C*)x, (C*)op->a1,
-                       op->row_start, op->row_dip, op->row_meta, (const C*)op->ccs, d, (const C*)op->tw[0], conj,
+    hipLaunchKernelGGL(k, dim3((3 * d.nrows + LN - 1) / LN, nrhs), dim3(LN * TPC), lds, st, (const C*)x, (C*)v.a1,
+                       op->row_start, op->row_dip, op->row_meta, (const C*)v.ccs, d, (const C*)op->tw[0], conj,
                        done);
   } else {
     auto k = k_pass5s<R, N, W>;
     DDA_TRY(set_lds(k, lds));
     const int nl = 3 * op->n_nonempty;
-    hipLaunchKernelGGL(k, dim3(nl > 0 ? (nl + LN - 1) / LN : 1, nrhs), dim3(LN * TPC), lds, st, (const C*)op->a1,
+    hipLaunchKernelGGL(k, dim3(nl > 0 ? (nl + LN - 1) / LN : 1, nrhs), dim3(LN * TPC), lds, st, (const C*)v.a1,
                        (const C*)x, (C*)y, op->row_list, op->n_nonempty, op->row_start, op->row_dip, op->row_meta,
-                       (const C*)op->ccs, d, (const C*)op->tw[0], conj, done);
+                       (const C*)v.ccs, d, (const C*)op->tw[0], conj, done);
   }
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
 }
 
 template <typename R, int N>
-dda_status launch_x_static(dda_op_s* op, bool inverse, const void* x, void* y, int conj, hipStream_t st,
-                           RhsFlags done, int nrhs) {
+dda_status launch_x_static(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
+                           hipStream_t st, RhsFlags done, int nrhs) {
   // default narrow for both precisions: half the LDS per wave, twice the waves
   // per CU (measured 52 -> 39 us at 256^3 FP32)
-  return op->tune.x_wide[inverse] > 0 ? launch_x_static_w<R, N, true>(op, inverse, x, y, conj, st, done, nrhs)
-                                      : launch_x_static_w<R, N, false>(op, inverse, x, y, conj, st, done, nrhs);
+  return op->tune.x_wide[inverse] > 0
+             ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs)
+             : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs);
 }
 
 // Lengths with a static plan; others use the run-time planned kernels.
@@ -1029,8 +1052,7 @@ dda_status launch_col_any(const void* in, void* out, const ColParams& cp, const 
 
 }  // namespace
 
-// The five passes of one matvec on `st`. done (nullable) lets a device-side
-// solver turn the launches into no-ops once it has finished.
+// Timing events of dda_op_profile, recycled between reads.
 static hipEvent_t pool_event(dda_op_s* op) {
   if (!op->prof_pool.empty()) {
     hipEvent_t e = op->prof_pool.back();
@@ -1063,34 +1085,26 @@ struct PassTimer {
   }
 };
 
+// Passes p0 .. p1 (0-based: X forward, Y forward, Z + tensor, Y inverse, X
+// inverse) of the matvec for right-hand sides r0 .. r0 + nrhs - 1 on `st`.
 template <typename R>
-dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done_ptr,
-                         int nrhs, int done_stride) {
+static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, RhsFlags done,
+                                int r0, int nrhs, int p0, int p1) {
   typedef typename CxT<R>::T C;
-  if (nrhs < 1 || nrhs > op->cap_rhs) {
-    set_error("matvec: number of right-hand sides exceeds the operator's capacity");
-    return DDA_INVALID_ARG;
-  }
-  if (nrhs > 65535 || 3 * op->dims.Bz > 65535) {
-    set_error("matvec: more than 65535 right-hand sides, or 3 * box z above 65535");
-    return DDA_NOT_SUPPORTED;
-  }
+  const MvView v = view_of(op, r0);
+  const MvDims& d = v.d;
+  const C* x = (const C*)xv + (size_t)r0 * 3 * d.S;
+  C* y = (C*)yv + (size_t)r0 * 3 * d.S;
+  if (done.done) done.done += (long long)r0 * done.stride;
+  C* a1 = (C*)v.a1;
+  C* a2 = (C*)v.a2;
+  const C* ccs = (const C*)v.ccs;
+  RhsFlags col_done = done;
+  col_done.per = 3 * d.Bz;
   // The Y passes launch one grid row per (RHS, component, z) column batch; a
   // launch holds at most 65535 rows, so larger blocks run as several launches
   // of whole right-hand sides.
-  const int rhs_per_col_launch = 65535 / (3 * op->dims.Bz);
-  RhsFlags done;
-  done.done = done_ptr;
-  done.stride = done_stride;
-  done.per = 1;
-  RhsFlags col_done = done;
-  col_done.per = 3 * op->dims.Bz;
-  const MvDims& d = op->dims;
-  const C* x = (const C*)xv;
-  C* y = (C*)yv;
-  C* a1 = (C*)op->a1;
-  C* a2 = (C*)op->a2;
-  const C* ccs = (const C*)op->ccs;
+  const int rhs_per_col_launch = 65535 / (3 * d.Bz);
 
   // pass 1 / pass 5 shapes
   const int lr = op->p1_rows;
@@ -1099,34 +1113,35 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   const int ppt_x = pts_x <= 8 * 256 ? 8 : 16;
   const size_t lds_x = (size_t)d.Gx * lds_stride(L) * sizeof(C);
   const FastDiv ldiv = make_fastdiv(L), ndiv = make_fastdiv(d.Gx);
-  if (op->static_x) {
+  if (p0 <= 0 && 0 <= p1) {
     PassTimer pt(op, st, 0);
-    switch (d.Gx) {
-#define X(N)                                                                \
-  case N:                                                                   \
-    DDA_TRY((launch_x_static<R, N>(op, false, x, nullptr, conj, st, done, nrhs))); \
+    if (op->static_x) {
+      switch (d.Gx) {
+#define X(N)                                                                         \
+  case N:                                                                            \
+    DDA_TRY((launch_x_static<R, N>(op, v, false, x, nullptr, conj, st, done, nrhs))); \
     break;
-      DDA_STATIC_SIZES(X)
+        DDA_STATIC_SIZES(X)
 #undef X
-    }
-  } else {
-    PassTimer pt(op, st, 0);
-    dim3 grid((d.nrows + lr - 1) / lr, nrhs);
-    if (ppt_x == 8) {
-      auto k = k_pass1<R, 8>;
-      DDA_TRY(set_lds(k, lds_x));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, x, a1, op->row_start, op->row_dip,
-                         op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+      }
     } else {
-      auto k = k_pass1<R, 16>;
-      DDA_TRY(set_lds(k, lds_x));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, x, a1, op->row_start, op->row_dip,
-                         op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+      dim3 grid((d.nrows + lr - 1) / lr, nrhs);
+      if (ppt_x == 8) {
+        auto k = k_pass1<R, 8>;
+        DDA_TRY(set_lds(k, lds_x));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, x, a1, op->row_start, op->row_dip,
+                           op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+      } else {
+        auto k = k_pass1<R, 16>;
+        DDA_TRY(set_lds(k, lds_x));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, x, a1, op->row_start, op->row_dip,
+                           op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv, (const C*)op->tw[0], conj, done);
+      }
+      DDA_TRY_HIP(hipGetLastError());
     }
-    DDA_TRY_HIP(hipGetLastError());
   }
   // pass 2: Y forward, A1 -> A2, batch = (c, z)
-  {
+  if (p0 <= 1 && 1 <= p1) {
     PassTimer pt(op, st, 1);
     ColParams cp;
     cp.ncols = d.Gx;
@@ -1136,56 +1151,57 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.By * d.Gx;
     cp.bstride_out = (long long)d.Gy * d.Gx;
-    for (int r0 = 0; r0 < nrhs; r0 += rhs_per_col_launch) {
-      const int k = std::min(rhs_per_col_launch, nrhs - r0);
-      const long long nb0 = 3ll * d.Bz * r0;
+    for (int q0 = 0; q0 < nrhs; q0 += rhs_per_col_launch) {
+      const int k = std::min(rhs_per_col_launch, nrhs - q0);
+      const long long nb0 = 3ll * d.Bz * q0;
       RhsFlags f = col_done;
-      if (f.done) f.done += (long long)r0 * f.stride;
+      if (f.done) f.done += (long long)q0 * f.stride;
       DDA_TRY((launch_col_any<R, false>(a1 + nb0 * cp.bstride_in, a2 + nb0 * cp.bstride_out, cp, op->plan[1],
                                         op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[0])));
     }
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
-  if (op->static_z) {
+  if (p0 <= 2 && 2 <= p1) {
     PassTimer pt(op, st, 2);
-    switch (d.Gz) {
-#define X(N)                                               \
-  case N:                                                  \
-    DDA_TRY((launch_pass3_static<R, N>(op, st, done, nrhs))); \
+    if (op->static_z) {
+      switch (d.Gz) {
+#define X(N)                                                         \
+  case N:                                                            \
+    DDA_TRY((launch_pass3_static<R, N>(op, v, st, done, nrhs))); \
     break;
-      DDA_STATIC_SIZES(X)
+        DDA_STATIC_SIZES(X)
 #undef X
-      default:
-        break;
-    }
-  } else {
-    PassTimer pt(op, st, 2);
-    const int T = op->col_t;
-    const int pts = d.Gz * T;
-    const size_t lds = (size_t)d.Gz * lds_stride(T) * sizeof(C);
-    const int ntx = (d.Gx + T - 1) / T;
-    const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
-    const int nper = (nwork + 7) / 8;
-    dim3 grid(8 * nper);
-    FastDiv tdiv = make_fastdiv(T);
-    if (pts <= 8 * 256) {
-      auto k = k_pass3<R, 8>;
-      DDA_TRY(set_lds(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], nwork, nper, nrhs, done);
-    } else if (pts <= 16 * 256) {
-      auto k = k_pass3<R, 16>;
-      DDA_TRY(set_lds(k, lds));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
-                         (const C*)op->tw[2], nwork, nper, nrhs, done);
+        default:
+          break;
+      }
     } else {
-      set_error("Z pass tile too large");
-      return DDA_NOT_SUPPORTED;
+      const int T = op->col_t;
+      const int pts = d.Gz * T;
+      const size_t lds = (size_t)d.Gz * lds_stride(T) * sizeof(C);
+      const int ntx = (d.Gx + T - 1) / T;
+      const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
+      const int nper = (nwork + 7) / 8;
+      dim3 grid(8 * nper);
+      FastDiv tdiv = make_fastdiv(T);
+      if (pts <= 8 * 256) {
+        auto k = k_pass3<R, 8>;
+        DDA_TRY(set_lds(k, lds));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 8)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
+                           (const C*)op->tw[2], nwork, nper, nrhs, done);
+      } else if (pts <= 16 * 256) {
+        auto k = k_pass3<R, 16>;
+        DDA_TRY(set_lds(k, lds));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts, 16)), lds, st, a2, (const C*)op->dhat, d, op->plan[2], tdiv,
+                           (const C*)op->tw[2], nwork, nper, nrhs, done);
+      } else {
+        set_error("Z pass tile too large");
+        return DDA_NOT_SUPPORTED;
+      }
+      DDA_TRY_HIP(hipGetLastError());
     }
-    DDA_TRY_HIP(hipGetLastError());
   }
   // pass 4: Y inverse, A2 -> A1 cropped to y < By
-  {
+  if (p0 <= 3 && 3 <= p1) {
     PassTimer pt(op, st, 3);
     ColParams cp;
     cp.ncols = d.Gx;
@@ -1195,44 +1211,92 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
     cp.pstride_out = d.Gx;
     cp.bstride_in = (long long)d.Gy * d.Gx;
     cp.bstride_out = (long long)d.By * d.Gx;
-    for (int r0 = 0; r0 < nrhs; r0 += rhs_per_col_launch) {
-      const int k = std::min(rhs_per_col_launch, nrhs - r0);
-      const long long nb0 = 3ll * d.Bz * r0;
+    for (int q0 = 0; q0 < nrhs; q0 += rhs_per_col_launch) {
+      const int k = std::min(rhs_per_col_launch, nrhs - q0);
+      const long long nb0 = 3ll * d.Bz * q0;
       RhsFlags f = col_done;
-      if (f.done) f.done += (long long)r0 * f.stride;
+      if (f.done) f.done += (long long)q0 * f.stride;
       DDA_TRY((launch_col_any<R, true>(a2 + nb0 * cp.bstride_in, a1 + nb0 * cp.bstride_out, cp, op->plan[1],
                                        op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[1])));
     }
   }
   // pass 5
-  if (op->static_x) {
+  if (p0 <= 4 && 4 <= p1) {
     PassTimer pt(op, st, 4);
-    switch (d.Gx) {
-#define X(N)                                                           \
-  case N:                                                              \
-    DDA_TRY((launch_x_static<R, N>(op, true, x, y, conj, st, done, nrhs))); \
+    if (op->static_x) {
+      switch (d.Gx) {
+#define X(N)                                                                    \
+  case N:                                                                       \
+    DDA_TRY((launch_x_static<R, N>(op, v, true, x, y, conj, st, done, nrhs))); \
     break;
-      DDA_STATIC_SIZES(X)
+        DDA_STATIC_SIZES(X)
 #undef X
-    }
-  } else {
-    PassTimer pt(op, st, 4);
-    dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1, nrhs);
-    if (ppt_x == 8) {
-      auto k = k_pass5<R, 8>;
-      DDA_TRY(set_lds(k, lds_x));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
-                         op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
-                         (const C*)op->tw[0], conj, done);
+      }
     } else {
-      auto k = k_pass5<R, 16>;
-      DDA_TRY(set_lds(k, lds_x));
-      hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
-                         op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
-                         (const C*)op->tw[0], conj, done);
+      dim3 grid((op->n_nonempty + lr - 1) / lr > 0 ? (op->n_nonempty + lr - 1) / lr : 1, nrhs);
+      if (ppt_x == 8) {
+        auto k = k_pass5<R, 8>;
+        DDA_TRY(set_lds(k, lds_x));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 8)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
+                           op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
+                           (const C*)op->tw[0], conj, done);
+      } else {
+        auto k = k_pass5<R, 16>;
+        DDA_TRY(set_lds(k, lds_x));
+        hipLaunchKernelGGL(k, grid, dim3(block_for(pts_x, 16)), lds_x, st, a1, x, y, op->row_list, op->n_nonempty,
+                           op->row_start, op->row_dip, op->row_meta, ccs, d, lr, op->plan[0], ldiv, ndiv,
+                           (const C*)op->tw[0], conj, done);
+      }
+      DDA_TRY_HIP(hipGetLastError());
     }
-    DDA_TRY_HIP(hipGetLastError());
   }
+  return DDA_OK;
+}
+
+// The second stream and the two events of a split block matvec, made on first use.
+static dda_status split_resources(dda_op_s* op) {
+  if (op->split_stream) return DDA_OK;
+  DDA_TRY_HIP(hipStreamCreateWithFlags(&op->split_stream, hipStreamNonBlocking));
+  for (auto& e : op->split_ev) DDA_TRY_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return DDA_OK;
+}
+
+// The five passes of one matvec on `st`. done (nullable) lets a device-side
+// solver turn the launches into no-ops once it has finished.
+//
+// A block of two or more right-hand sides with the split enabled runs as two
+// halves on two streams, staggered by two passes: the second half starts once
+// the first has finished its Y pass, so the Z pass of each half (bound by its
+// arithmetic) runs beside the memory-bound X and Y passes of the other. The
+// fork and the join are event waits, which a stream capture records as graph
+// edges. Per-pass profiling keeps the block on one stream.
+template <typename R>
+dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done_ptr,
+                         int nrhs, int done_stride) {
+  if (nrhs < 1 || nrhs > op->cap_rhs) {
+    set_error("matvec: number of right-hand sides exceeds the operator's capacity");
+    return DDA_INVALID_ARG;
+  }
+  if (nrhs > 65535 || 3 * op->dims.Bz > 65535) {
+    set_error("matvec: more than 65535 right-hand sides, or 3 * box z above 65535");
+    return DDA_NOT_SUPPORTED;
+  }
+  RhsFlags done;
+  done.done = done_ptr;
+  done.stride = done_stride;
+  done.per = 1;
+  const bool split = op->split_rhs && nrhs >= 2 && !op->prof;
+  if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4);
+  DDA_TRY(split_resources(op));
+  const int h = (nrhs + 1) / 2;
+  hipStream_t aux = op->split_stream;
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 0, 1)));
+  DDA_TRY_HIP(hipEventRecord(op->split_ev[0], st));
+  DDA_TRY_HIP(hipStreamWaitEvent(aux, op->split_ev[0], 0));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 2, 4)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, aux, done, h, nrhs - h, 0, 4)));
+  DDA_TRY_HIP(hipEventRecord(op->split_ev[1], aux));
+  DDA_TRY_HIP(hipStreamWaitEvent(st, op->split_ev[1], 0));
   return DDA_OK;
 }
 
@@ -1538,6 +1602,12 @@ extern "C" uint32_t dda_fft_fit(uint32_t x, uint32_t override_, int support_non_
 }
 
 static void free_op(dda_op_s* op) {
+  if (op->split_stream) {
+    (void)hipStreamSynchronize(op->split_stream);
+    (void)hipStreamDestroy(op->split_stream);
+  }
+  for (auto e : op->split_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& r : op->prof_recs) {
     (void)hipEventDestroy(r.b);
     (void)hipEventDestroy(r.e);
@@ -1655,6 +1725,7 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
   const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
   op->static_z = static_z_fits(d.Gx, d.Gy, d.Gz, csz) && !std::getenv("DDA_GENERIC_FFT");
   op->static_x = has_static_plan(d.Gx) && !std::getenv("DDA_GENERIC_FFT");
+  op->split_rhs = !std::getenv("DDA_NO_SPLIT");
 
   // Launch shapes: X passes hold ~2048 points per work-group; column passes
   // take 128-byte row segments (8 complex doubles, 16 complex floats).
@@ -2311,7 +2382,8 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
           switch (d.Gx) {
 #define X(N) \
   case N:    \
-    return launch_x_static<R, N>(op, pass == 4, xb, xb + 3ull * d.S * csz * nrhs, 0, st, RhsFlags(), nrhs);
+    return launch_x_static<R, N>(op, view_of(op, 0), pass == 4, xb, xb + 3ull * d.S * csz * nrhs, 0, st, RhsFlags(), \
+                                 nrhs);
             DDA_STATIC_SIZES(X)
 #undef X
             default:
@@ -2324,7 +2396,7 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
           switch (d.Gz) {
 #define X(N) \
   case N:    \
-    return launch_pass3_static<R, N>(op, st, RhsFlags(), nrhs);
+    return launch_pass3_static<R, N>(op, view_of(op, 0), st, RhsFlags(), nrhs);
             DDA_STATIC_SIZES(X)
 #undef X
             default:

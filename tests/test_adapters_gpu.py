@@ -35,7 +35,7 @@ def test_cpp_adapters_match_oracle(tmp_path):
     r = subprocess.run([BIN, str(case), str(out)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "adapter_check ok" in r.stdout and "batch ok" in r.stdout
-    assert r.stdout.count("+-") == 2  # the profiling runs of the float and the double solver
+    assert r.stdout.count("profiling run:") == 2  # the float and the double solver
     n = params.vec_size
     data = np.fromfile(out, np.complex128, 4 * n)
     it, it32 = (int(v) for v in np.fromfile(out, np.uint64, 2, offset=4 * n * 16))

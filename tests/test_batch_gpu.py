@@ -174,3 +174,42 @@ def test_batched_start_vectors(ctx, mixed):
     pb, infos = s.get_pol_vecs(einc, 1e-10, start=p0)
     for r in range(len(ccs)):
         assert np.array_equal(pb[r], alone[r]), r
+
+
+@pytest.mark.parametrize("precision,mode", [(64, "fp64"), (64, "mixed")])
+def test_split_block_matches_one_stream(ctx, precision, mode, monkeypatch):
+    """A block of right-hand sides runs as two halves on two streams (the Z
+    pass of one beside the X and Y passes of the other); the block matvec and
+    a block solve (whose steady iterations replay a captured graph holding the
+    fork and join) give bitwise what the block gives on one stream, over
+    repeated solves."""
+    print("split block", mode, flush=True)
+    g, params, einc, ccs = _setup("two_materials", 3, 17)  # 6 right-hand sides
+    k, n = len(ccs), params.vec_size
+    monkeypatch.setenv("DDA_NO_SPLIT", "1")
+    op1 = _op(ctx, g, params, precision)
+    monkeypatch.delenv("DDA_NO_SPLIT")
+    op2 = _op(ctx, g, params, precision)
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex128)
+    for r in range(k):
+        x[r, params.nv:params.vec_stride] = 0
+    xb = D.DeviceVector(ctx, k * n, precision)
+    xb.upload(x.reshape(-1))
+    outs = []
+    for op in (op1, op2):
+        op.set_couple_constants_batch(ccs)
+        yb = D.DeviceVector(ctx, k * n, precision)
+        op.apply_batch(xb, yb, k)
+        outs.append(yb.download())
+    assert np.array_equal(outs[0], outs[1])
+    sols = []
+    for op in (op1, op2):
+        s = D.GpuBicgStab(op, max_iter=2000, mode=mode)
+        runs = [s.get_pol_vecs(einc, 1e-9) for _ in range(3)]
+        for p, infos in runs:
+            assert all(i["status"] == D.OK for i in infos)
+            assert np.array_equal(p, runs[0][0])
+        sols.append(runs[0][0])
+        s.close()
+    assert np.array_equal(sols[0], sols[1])
