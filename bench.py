@@ -182,7 +182,7 @@ def c3_bench(args, D, P, np, ctx):
     return out
 
 
-def large_bench(args, D, P, np):
+def large_matvec(args, D, P, np, ctx, solves=False):
     """The largest single-GPU grid: sphere 512 dipoles across (~70M dipoles,
     1024^3), m = 1.6+0.05i, |m| kd = 0.5. FP32 and FP64 matvec rates against
     B_mv, the device memory held, and optionally mixed BiCGStab / QMR-CS to
@@ -195,47 +195,54 @@ def large_bench(args, D, P, np):
     params = P.DDAParams(g, 2 * math.pi * d / kd)
     t_geom = time.perf_counter() - t0
     print("large: nv=%d grid=%s geometry %.1f s" % (params.nv, params.grid, t_geom), file=sys.stderr, flush=True)
-    ctx = D.Context(0)
     prop, pol1, pol2 = P.beam_pols((0, 0, 1))
     cc, cs = P.couple_constants(params, prop, pol1)
     out = {"workload": "sphere 512 dipoles across (nv=%d), m=1.6+0.05i, |m|kd=0.5, grid %s"
                        % (params.nv, "x".join(map(str, params.grid)))}
-    op64 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, d, params.wave_num, precision=64, grid=params.grid)
-    op64.set_couple_constants(cs)
+    rows = np.unique(g.pos[:, 2].astype(np.int64) * g.box[1] + g.pos[:, 1]).size
+    xh = P.random_dipole_vector(params)
+    op64 = None
     for name, prec in (("fp32", 32), ("fp64", 64)):
         t0 = time.perf_counter()
-        op = op64 if prec == 64 else D.GpuMatVec(ctx, g.pos, g.mat, g.box, d, params.wave_num, precision=32,
-                                                 grid=params.grid)
+        op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, d, params.wave_num, precision=prec, grid=params.grid)
         op.set_couple_constants(cs)
         t_create = time.perf_counter() - t0
         tuning = op.tune(1) if not args.no_tune else "not tuned"
         x = D.DeviceVector(ctx, params.vec_size, prec)
         y = D.DeviceVector(ctx, params.vec_size, prec)
-        x.upload(P.random_dipole_vector(params))
+        x.upload(xh)
         op.apply(x, y)
         ctx.synchronize()
+        # the rate as a caller sees it (no events), then per-pass event times
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.apply(x, y)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps * 1e3
         op.profile(True)
-        for _ in range(5):
+        for _ in range(reps):
             op.apply(x, y)
         ms, cnt, _ = op.profile_read()
         op.profile(False)
         pav = [ms[i] / max(cnt[i], 1) for i in range(5)]
         mv = float(sum(pav))
         c = 8 if prec == 32 else 16
-        rows = np.unique(g.pos[:, 2].astype(np.int64) * g.box[1] + g.pos[:, 1]).size
         pb = pass_bytes(params.nv, params.vec_stride, g.box, params.grid, rows, c)
         bmv = b_mv(params.nv, g.box, params.grid, c)
-        out[name] = {"matvec_ms": mv, "pass_avg_us": [1e3 * v for v in pav],
+        out[name] = {"matvec_ms": wall, "matvec_ms_events": mv, "pass_avg_us": [1e3 * v for v in pav],
                      "pass_gbs": [pb[i] / (pav[i] * 1e-3) / 1e9 for i in range(5)],
-                     "gbs_vs_bmv": bmv / (mv * 1e-3) / 1e9, "frac_vs_bmv": bmv / (mv * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "gbs_algorithmic": sum(pb) / (mv * 1e-3) / 1e9, "device_bytes": op.device_bytes,
-                     "create_s": t_create, "tuning": tuning}
+                     "gbs_vs_bmv": bmv / (wall * 1e-3) / 1e9, "frac_vs_bmv": bmv / (wall * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "gbs_algorithmic": sum(pb) / (wall * 1e-3) / 1e9, "device_bytes": op.device_bytes,
+                     "create_s": t_create, "tuning": tuning, "variants": op.variants()}
         print("large %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
         x.close()
         y.close()
-        if prec == 32:
+        if prec == 64:
+            op64 = op
+        else:
             op.close()
-    if args.large_solves:
+    if solves:
         einc = P.einc_plane(params, prop, pol1)
         out["solves"] = {}
         for kind in args.large_solvers.split(","):
@@ -256,7 +263,13 @@ def large_bench(args, D, P, np):
             print("large solve %s: %s" % (kind, json.dumps(res)), file=sys.stderr, flush=True)
             s.close()
     out["device_bytes_total"] = op64.device_bytes + out["fp32"]["device_bytes"]
-    print(json.dumps(out))
+    op64.close()
+    return out
+
+
+def large_bench(args, D, P, np):
+    ctx = D.Context(0)
+    print(json.dumps(large_matvec(args, D, P, np, ctx, solves=args.large_solves)))
     return 0
 
 
@@ -280,34 +293,45 @@ def _plain(o, path="line"):
 def serving_bench(args, D):
     """The serving stream: jobs of random shape (sphere / box / ellipsoid, largest
     extent 16..96 dipoles), Re m in 1.2..2, Im m in 0..0.2, |m| kd = 0.5, random
-    orientation, both polarisations, mixed BiCGStab to eps. First all jobs at
-    once (capacity), then a fresh seeded stream with Poisson arrivals at 0.9x
-    that capacity; latency = completion - submission."""
+    orientation, both polarisations, mixed BiCGStab to eps, 4 slots. Capacity
+    from a burst of jobs submitted at once; then fresh seeded streams with
+    Poisson arrivals at 0.5x, 0.9x and 1.2x that capacity; latency = completion
+    - submission. The streams are generated on host threads ahead of use."""
+    from concurrent.futures import ThreadPoolExecutor
     from dda_amd import serving as S
-    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
-    try:
-        S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
-        burst = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=42))
-        rate = 0.9 * burst["jobs"] / burst["makespan_s"]
-        stream = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=43), rate=rate)
-        stats = srv.stats()
-    finally:
-        srv.close()
-    # the same burst with different particles of one grid packed into one solve
+    loads = (0.5, 0.9, 1.2)
+    out = {"workload": "seeded streams of %d jobs per load (burst of %d for capacity), sphere/box/ellipsoid, extent "
+                       "U{16..96}, Re m U(1.2,2), Im m U(0,0.2), |m|kd=0.5, random orientation, 2 polarisations, "
+                       "mixed BiCGStab to %g, 4 slots" % (args.serve_jobs, args.serve_burst, args.eps)}
+    with ThreadPoolExecutor(4) as ex:
+        f_burst = ex.submit(S.make_stream, args.serve_burst, 42)
+        f_streams = [ex.submit(S.make_stream, args.serve_jobs, 43 + i) for i in range(len(loads))]
+        burst_jobs = f_burst.result()
+        srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
+        try:
+            S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
+            burst = S.run_stream(srv, burst_jobs)
+            cap = burst["jobs"] / burst["makespan_s"]
+            out["capacity"] = dict(burst, capacity_jobs_per_s=cap)
+            for load, f in zip(loads, f_streams):
+                jobs = f.result()
+                r = S.run_stream(srv, jobs, rate=load * cap)
+                out["poisson_%.1fx" % load] = dict(r, rate_jobs_per_s=load * cap)
+                print("serving %.1fx: %s" % (load, json.dumps(r)), file=sys.stderr, flush=True)
+                del jobs
+            out["server"] = srv.stats()
+        finally:
+            srv.close()
+    # the burst again with different particles of one grid packed into one solve
     srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))
-        burst_pg = S.run_stream(srv, S.make_stream(args.serve_jobs, seed=42))
+        out["capacity_pack_grids"] = S.run_stream(srv, burst_jobs)
     finally:
         srv.close()
-    strip = lambda r: {k: v for k, v in r.items()}
-    return {"workload": "%d jobs per run, sphere/box/ellipsoid, extent U{16..96}, Re m U(1.2,2), Im m U(0,0.2), "
-                        "|m|kd=0.5, random orientation, 2 polarisations, mixed BiCGStab to %g, 4 slots"
-                        % (args.serve_jobs, args.eps),
-            "capacity": strip(burst), "capacity_pack_grids": strip(burst_pg),
-            "poisson_0.9x": dict(strip(stream), rate_jobs_per_s=rate),
-            "p50_ms": stream["p50_ms"], "p99_ms": stream["p99_ms"], "solves_per_s": stream["solves_per_s"],
-            "server": stats}
+    ref = out["poisson_0.9x"]
+    out.update({"p50_ms": ref["p50_ms"], "p99_ms": ref["p99_ms"], "solves_per_s": ref["solves_per_s"]})
+    return out
 
 
 def main():
@@ -334,8 +358,11 @@ def main():
     ap.add_argument("--large-max-iter", type=int, default=20000)
     ap.add_argument("--orient-dirs", type=int, default=8,
                     help="directions of the extra orientation-batch measurement (0 = skip)")
-    ap.add_argument("--serve-jobs", type=int, default=24,
-                    help="jobs of the seeded serving stream (0 = skip the serving measurement)")
+    ap.add_argument("--serve-jobs", type=int, default=200,
+                    help="jobs of each seeded Poisson serving stream (0 = skip the serving measurement)")
+    ap.add_argument("--serve-burst", type=int, default=48, help="jobs of the capacity burst")
+    ap.add_argument("--no-large-grid", action="store_true",
+                    help="skip the 1024^3 (sphere 512 across) FP32 / FP64 matvec figures")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
                     help="instead of solves: one 512 MiB device copy (counter calibration), then K FP32 and "
                          "K FP64 matvecs; prints per-pass times (for rocprofv3 counter runs)")
@@ -599,6 +626,14 @@ def main():
         except Exception as e:  # reported, not fatal: the headline is the solve rate of the 128-across sphere
             serving = {"error": str(e)}
 
+    large = None
+    if not args.no_large_grid:
+        try:
+            op.close()  # the headline operator's memory back before the 1024^3 operators
+            large = large_matvec(args, D, P, np, ctx)
+        except Exception as e:  # reported, not fatal
+            large = {"error": str(e)}
+
     info = ctx.info()
     # what a plain device-to-device copy sustains on this GPU (1 GiB, read +
     # write counted): the practical ceiling of a streaming pass
@@ -662,6 +697,7 @@ def main():
         "orientation_batch": orient,
         "c3_batch": c3,
         "serving": serving,
+        "large_grid": large,
     }
     print(json.dumps(_plain(line)))
     return 0

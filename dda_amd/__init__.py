@@ -36,14 +36,15 @@ EXPORTS = [
     "dda_malloc", "dda_free", "dda_memcpy_h2d", "dda_memcpy_d2h", "dda_memcpy_d2d", "dda_fft_fit",
     "dda_fft_fit_static", "dda_grid_plan",
     "dda_op_create", "dda_op_destroy", "dda_op_info", "dda_op_set_cc_sqrt", "dda_op_set_cc_sqrt_batch",
-    "dda_op_apply", "dda_op_apply_batch", "dda_op_tune",
+    "dda_op_apply", "dda_op_apply_batch", "dda_op_tune", "dda_op_get_variants", "dda_op_set_variants",
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
     "dda_solver_set_inner_floor", "dda_solver_profile_read", "dda_solver_profiling_run",
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
-    "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_incident_plane", "dda_cross_sections",
+    "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_couple_constants_diag", "dda_incident_plane",
+    "dda_cross_sections",
     "dda_far_field", "dda_jones", "dda_op_create_pack", "dda_op_pack_info", "dda_op_pack_map",
     "dda_op_set_rhs_members",
 ]
@@ -138,6 +139,8 @@ def lib():
         L.dda_op_tune.argtypes = [vp, i32, ctypes.c_char_p, sz]
         L.dda_op_apply_host.argtypes = [vp, dp, dp, i32]
         L.dda_op_get_dmatrix.argtypes = [vp, dp]
+        L.dda_op_get_variants.argtypes = [vp, ctypes.POINTER(i32)]
+        L.dda_op_set_variants.argtypes = [vp, ctypes.POINTER(i32)]
         L.dda_op_profile.argtypes = [vp, i32]
         L.dda_op_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_create.argtypes = [vp, i32, u64, u64, pp]
@@ -160,6 +163,7 @@ def lib():
         L.dda_server_destroy.argtypes = [vp]
         L.dda_solver_solve_batch_dev.argtypes = [vp, vp, i32, dbl, vp, vp, ctypes.POINTER(SolveInfo)]
         L.dda_couple_constants.argtypes = [i32, u32, dp, dbl, dbl, dp, dp, dp, dp]
+        L.dda_couple_constants_diag.argtypes = [i32, u32, dp, dbl, dbl, dp, dp, dp, dp]
         L.dda_incident_plane.argtypes = [vp, i32, dp, dp, vp]
         L.dda_cross_sections.argtypes = [vp, vp, vp, i32, dp, dp, dp]
         L.dda_far_field.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int), dp, i32, dp]
@@ -384,6 +388,15 @@ class GpuMatVec:
         buf = ctypes.create_string_buffer(512)
         _check(lib().dda_op_tune(self.handle, int(nrhs), buf, 512), "dda_op_tune")
         return buf.value.decode()
+
+    def variants(self, values=None):
+        """Kernel variant per pass (Y fwd, Z, Y inv, X fwd, X inv); with `values`, pins them first."""
+        a = (ctypes.c_int32 * 5)()
+        if values is not None:
+            a[:] = [int(v) for v in values]
+            _check(lib().dda_op_set_variants(self.handle, a), "dda_op_set_variants")
+        _check(lib().dda_op_get_variants(self.handle, a), "dda_op_get_variants")
+        return list(a)
 
     def dmatrix(self):
         """The device tensor in the reference layout [Gx, Gz, Gy, 6] (1/m^3)."""
@@ -753,15 +766,24 @@ class Server:
 
 
 def couple_constants(materials, grid_unit, kd, prop, pol, kind="ldr"):
-    """dda_couple_constants -> (cc, cc_sqrt), each [nmat, 3] complex."""
-    m = np.ascontiguousarray(np.asarray(materials, np.complex128).reshape(-1))
-    cc = np.zeros((m.size, 3), np.complex128)
-    cs = np.zeros((m.size, 3), np.complex128)
+    """dda_couple_constants -> (cc, cc_sqrt), each [nmat, 3] complex. A material
+    given as three values (the diagonal of an anisotropic index) goes through
+    dda_couple_constants_diag."""
+    aniso = any(np.ndim(v) > 0 for v in materials)
+    if aniso:
+        m = np.ascontiguousarray([np.broadcast_to(np.asarray(v, np.complex128), (3,)) for v in materials])
+        n = m.shape[0]
+        fn, name = lib().dda_couple_constants_diag, "dda_couple_constants_diag"
+    else:
+        m = np.ascontiguousarray(np.asarray(materials, np.complex128).reshape(-1))
+        n = m.size
+        fn, name = lib().dda_couple_constants, "dda_couple_constants"
+    cc = np.zeros((n, 3), np.complex128)
+    cs = np.zeros((n, 3), np.complex128)
     pr = np.ascontiguousarray(prop, np.float64)
     pl = np.ascontiguousarray(pol, np.float64)
-    _check(lib().dda_couple_constants(POL_KINDS[kind], m.size, _dp(m.view(np.float64)), float(grid_unit), float(kd),
-                                      _dp(pr), _dp(pl), _dp(cc.view(np.float64)), _dp(cs.view(np.float64))),
-           "dda_couple_constants")
+    _check(fn(POL_KINDS[kind], n, _dp(m.view(np.float64)), float(grid_unit), float(kd), _dp(pr), _dp(pl),
+              _dp(cc.view(np.float64)), _dp(cs.view(np.float64))), name)
     return cc, cs
 
 
@@ -807,3 +829,35 @@ def jones(op, pol_dev, symmetric, prop, pol1, pol2, ntheta, theta0, dtheta, nphi
                            float(theta0), float(dtheta), int(nphi), float(phi0), float(dphi),
                            _dp(out.view(np.float64))), "dda_jones")
     return out
+
+
+def solve_particle(solver, params, prop, eps, kind="ldr", symmetric=None, pol_dev=None):
+    """Both plane-wave polarisations of one particle, as the reference's
+    createResOutput solves them (DDA/DDAMain.cpp:380-410, 452-470): for a
+    rotation-symmetric target (problem.rotation_symmetric; `symmetric`
+    overrides the rule) polarisation 1 alone is solved, polarisation 2's cross
+    sections are polarisation 1's and its solution is polarisation 1's turned
+    a quarter turn about z (jones(..., symmetric=1) applies that); otherwise
+    both are solved as one 2-RHS block. Incident fields are built on the
+    device. Returns a dict: symmetric, nrhs, pol_dev (DeviceVector of nrhs
+    solutions), infos, cext [2], cabs [2], prop, pol1, pol2."""
+    from . import problem as P
+    op = solver.op
+    prop_n, pol1, pol2 = P.beam_pols(prop)
+    sym = P.rotation_symmetric(params.geometry, prop_n) if symmetric is None else bool(symmetric)
+    pols = [pol1] if sym else [pol1, pol2]
+    ccs = [P.couple_constants(params, prop_n, p, kind) for p in pols]
+    k = len(pols)
+    op.set_couple_constants_batch(np.array([cs for _, cs in ccs]))
+    e_dev = incident_plane(op, [prop_n] * k, pols)
+    if pol_dev is None:
+        pol_dev = DeviceVector(op.ctx, k * op.vec_size, 64)
+    try:
+        infos = solver.solve_device(e_dev, pol_dev, k, eps)
+        cext, cabs = cross_sections(op, pol_dev, e_dev, k, np.array([cc for cc, _ in ccs]))
+    finally:
+        e_dev.close()
+    if sym:
+        cext, cabs = np.array([cext[0], cext[0]]), np.array([cabs[0], cabs[0]])
+    return {"symmetric": sym, "nrhs": k, "pol_dev": pol_dev, "infos": infos, "cext": cext, "cabs": cabs,
+            "prop": prop_n, "pol1": pol1, "pol2": pol2}

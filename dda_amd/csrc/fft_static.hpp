@@ -245,6 +245,15 @@ DDA_PLAN(1024, false, 8, 8, 8, 4, 4)
 DDA_PLAN(1024, true, 16, 16, 16, 4)
 #undef DDA_PLAN
 
+// A plan given by its points per thread and radix order directly (the Z
+// pass's long-column variants: 1024 = 32 x 32 holds 32 points per thread and
+// needs a single LDS exchange per transform).
+template <int N, int P, typename L> struct PlanOf {
+  static constexpr bool exists = true;
+  static constexpr int PPT = P;
+  typedef L Fwd;
+};
+
 template <typename L> struct Reverse;
 template <> struct Reverse<RadixList<>> { typedef RadixList<> type; };
 template <int R, int... Rs> struct Reverse<RadixList<R, Rs...>> {

@@ -141,7 +141,8 @@ struct dda_op_s {
   int p1_rows = 1;   // rows per work-group in the X passes
   int col_t = 8;     // columns per work-group in the Y/Z passes
   // Kernel variants (plan width of the static column passes: -1 default,
-  // 0 narrow, 1 wide), chosen by dda_op_tune.
+  // 0 narrow, 1 wide; the Z pass at 1024 points also 2, 3: the 32 x 32 plan
+  // over 16 or 8 columns), chosen by dda_op_tune.
   struct Tune {
     int col_wide[2] = {-1, -1};  // forward / inverse Y pass
     int p3_wide = -1;            // Z pass with the tensor product

@@ -900,12 +900,10 @@ struct Core {
   }
   std::vector<const void*> graph_signature() const {
     std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat,
-                                  reinterpret_cast<const void*>((intptr_t)nrhs),
-                                  reinterpret_cast<const void*>((intptr_t)(op->tune.col_wide[0] * 4 +
-                                                                           op->tune.col_wide[1] * 16 +
-                                                                           op->tune.p3_wide * 64 +
-                                                                           op->tune.x_wide[0] * 256 +
-                                                                           op->tune.x_wide[1] * 1024 + 4096))};
+                                  reinterpret_cast<const void*>((intptr_t)nrhs)};
+    for (int t : {op->tune.col_wide[0], op->tune.col_wide[1], op->tune.p3_wide, op->tune.x_wide[0],
+                  op->tune.x_wide[1]})
+      g.push_back(reinterpret_cast<const void*>((intptr_t)t));
     for (C* p : v) g.push_back(p);
     return g;
   }

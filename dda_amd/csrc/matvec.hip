@@ -383,8 +383,9 @@ template <bool NT> __device__ __forceinline__ void cstore(double2* p, double2 v)
 // flight), other FP32 plans at 4, FP64 left to the compiler. At 1024 points
 // the 16-point plan runs 1024-thread work-groups, which fit a CU only at 4
 // waves per SIMD (128 VGPRs, a few spilled).
+// 32-point plans (FP32) hold 192 VGPRs of data: 2 waves per SIMD (256).
 template <typename R, int N, int PPT> struct P3Waves {
-  static constexpr int value = sizeof(R) == 4 ? (PPT >= 16 && N < 1024 ? 3 : 4) : 1;
+  static constexpr int value = sizeof(R) == 4 ? (PPT >= 32 ? 2 : PPT >= 16 && N < 1024 ? 3 : 4) : 1;
 };
 
 // Pass 3 with a static plan of length N = Gz (see k_pass3 for the mapping).
@@ -395,14 +396,15 @@ template <typename R, int N, int PPT> struct P3Waves {
 // and are not stored) and spans at most N / PPT planes (it stays below 4 GiB up
 // to 1024^3). The tensor is read through one buffer per component with a lane
 // offset per point.
-template <typename R, int N, int T, bool W>
-__global__ __launch_bounds__(T * (N / StaticPlan<N, W>::PPT))
-__attribute__((amdgpu_waves_per_eu(P3Waves<R, N, StaticPlan<N, W>::PPT>::value))) void k_pass3s(typename CxT<R>::T* a2,
+// PL is the plan (StaticPlan<N, W> or a PlanOf variant).
+template <typename R, int N, int T, typename PL>
+__global__ __launch_bounds__(T * (N / PL::PPT))
+__attribute__((amdgpu_waves_per_eu(P3Waves<R, N, PL::PPT>::value))) void k_pass3s(typename CxT<R>::T* a2,
                                                                               const typename CxT<R>::T* __restrict__ dhat,
                                                 MvDims d, const typename CxT<R>::T* __restrict__ tw, int nwork,
                                                 int nper, int nrhs, RhsFlags fl) {
   typedef typename CxT<R>::T C;
-  typedef StaticPlan<N, W> P;
+  typedef PL P;
   constexpr int PPT = P::PPT;
   constexpr int TPC = N / PPT;
   typedef typename P::Fwd F;
@@ -452,7 +454,8 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, N, StaticPlan<N, W>::PPT>::value))
   // Bz <= N/2 (the grid holds at least twice the box): the forward transforms
   // skip the zero upper half and the inverse ones never form it; two-stage
   // plans form the second-stage twiddle powers once for all six transforms
-  constexpr bool PRE = TwoStage<N, PPT, F>::value && sizeof(R) == 4;  // FP64: no registers to spare
+  // (FP64 and 32-point plans: no registers to spare)
+  constexpr bool PRE = TwoStage<N, PPT, F>::value && sizeof(R) == 4 && PPT <= 16;
   C wp[PRE ? PPT - 1 : 1];
   if constexpr (PRE) twiddle_powers<PPT>(wp, t, tw);
   const C* twp = PRE ? wp : tw;
@@ -940,9 +943,26 @@ inline MvView view_of(const dda_op_s* op, int r0) {
   return v;
 }
 
+template <typename R, int N, typename PL, int T>
+dda_status launch_pass3_plan(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
+  typedef typename CxT<R>::T C;
+  constexpr int threads = T * (N / PL::PPT);
+  constexpr int lp = (T & 1) ? T : T + 1;
+  const MvDims& d = v.d;
+  const size_t lds = (size_t)N * lp * sizeof(C);
+  const int ntx = (d.Gx + T - 1) / T;
+  const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
+  const int nper = (nwork + 7) / 8;
+  auto k = k_pass3s<R, N, T, PL>;
+  DDA_TRY(set_lds(k, lds));
+  hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)v.a2, (const C*)op->dhat, d,
+                     (const C*)op->tw[2], nwork, nper, nrhs, done);
+  DDA_TRY_HIP(hipGetLastError());
+  return DDA_OK;
+}
+
 template <typename R, int N, bool W>
 dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
-  typedef typename CxT<R>::T C;
   // FP32 1024-point columns: 16 columns (128-byte segments) in a 1024-thread
   // work-group at 128 VGPRs, the CU's whole register file (17.5 ms at 1024^3,
   // against 22.7 ms for two 512-thread work-groups of 8 columns)
@@ -951,25 +971,27 @@ dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, 
   // 781 us per launch
   constexpr int T = static_cols<R, N, W, (N == 1024 && sizeof(R) == 4 && W) ? 1024 : 512,
                                 (sizeof(R) == 4 && N <= 64) ? 32 : 0>();
-  constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
-  constexpr int lp = (T & 1) ? T : T + 1;
-  const MvDims& d = v.d;
-  const size_t lds = (size_t)N * lp * sizeof(C);
-  const int ntx = (d.Gx + T - 1) / T;
-  const int nwork = (d.Gy / 2 + 1) * ((ntx + 1) / 2) * 4 * nrhs;
-  const int nper = (nwork + 7) / 8;
-  auto k = k_pass3s<R, N, T, W>;
-  DDA_TRY(set_lds(k, lds));
-  hipLaunchKernelGGL(k, dim3(8 * nper), dim3(threads), lds, st, (C*)v.a2, (const C*)op->dhat, d,
-                     (const C*)op->tw[2], nwork, nper, nrhs, done);
-  DDA_TRY_HIP(hipGetLastError());
-  return DDA_OK;
+  return launch_pass3_plan<R, N, StaticPlan<N, W>, T>(op, v, st, done, nrhs);
 }
+
+// Z-pass variants (Tune::p3_wide): -1 the precision's default plan width,
+// 0 narrow, 1 wide; at 1024 points also 2 and 3, the 32 x 32 plan (32 points
+// per thread, one LDS exchange per transform, 2 waves per SIMD, FP32) over 16
+// columns (512 threads, 128-byte segments, one work-group per CU by LDS) or
+// 8 columns (256 threads, 64-byte segments, two work-groups per CU).
+// FP32 only: in FP64 the 32-point plan's data alone needs 384 VGPRs.
+typedef PlanOf<1024, 32, RadixList<32, 32>> Plan1024x32;
+inline int p3_variants(int n, int prec) { return n == 1024 && prec == 32 ? 4 : 2; }
 
 template <typename R, int N>
 dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
-  return plan_wide<R>(op->tune.p3_wide) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
-                                        : launch_pass3_static_w<R, N, false>(op, v, st, done, nrhs);
+  const int var = op->tune.p3_wide;
+  if constexpr (N == 1024 && sizeof(R) == 4) {
+    if (var == 2) return launch_pass3_plan<R, N, Plan1024x32, 16>(op, v, st, done, nrhs);
+    if (var == 3) return launch_pass3_plan<R, N, Plan1024x32, 8>(op, v, st, done, nrhs);
+  }
+  return plan_wide<R>(var) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
+                           : launch_pass3_static_w<R, N, false>(op, v, st, done, nrhs);
 }
 
 template <typename R, int N, bool W>
@@ -2372,8 +2394,9 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   for (int pass = 0; pass < 5; pass++) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
-    double t[2] = {0, 0};
-    for (int w = 0; w < 2; w++) {
+    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : 2;
+    double t[4] = {0, 0, 0, 0};
+    for (int w = 0; w < nvar; w++) {
       dda_status rc;
       if (pass >= 3) {
         op->tune.x_wide[pass - 3] = w;
@@ -2427,7 +2450,9 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
         return rc;
       }
     }
-    const int pick = t[1] <= t[0] ? 1 : 0;
+    int pick = t[1] <= t[0] ? 1 : 0;
+    for (int w = 2; w < nvar; w++)
+      if (t[w] < t[pick]) pick = w;
     if (pass == 1)
       best.p3_wide = pick;
     else if (pass >= 3)
@@ -2435,8 +2460,15 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
     else
       best.col_wide[pass == 2] = pick;
     static const char* names[5] = {"Y forward", "Z + tensor", "Y inverse", "X forward", "X inverse"};
-    std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us -> %s", report->empty() ? "" : "; ",
-                  names[pass], t[0], t[1], pick ? "wide" : "narrow");
+    static const char* vnames[4] = {"narrow", "wide", "32x32/16 cols", "32x32/8 cols"};
+    std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us", report->empty() ? "" : "; ", names[pass],
+                  t[0], t[1]);
+    *report += buf;
+    for (int w = 2; w < nvar; w++) {
+      std::snprintf(buf, sizeof buf, ", %s %.1f us", vnames[w], t[w]);
+      *report += buf;
+    }
+    std::snprintf(buf, sizeof buf, " -> %s", vnames[pick]);
     *report += buf;
     op->tune = best;
   }
@@ -2476,6 +2508,39 @@ extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t re
     std::strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = 0;
   }
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_get_variants(dda_op_t op, int variants[5]) {
+  if (!op || !variants) {
+    set_error("dda_op_get_variants: null argument");
+    return DDA_INVALID_ARG;
+  }
+  const dda_op_s::Tune& t = op->tune;
+  const int v[5] = {t.col_wide[0], t.p3_wide, t.col_wide[1], t.x_wide[0], t.x_wide[1]};
+  for (int i = 0; i < 5; i++) variants[i] = v[i];
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[5]) {
+  if (!op || !variants) {
+    set_error("dda_op_set_variants: null argument");
+    return DDA_INVALID_ARG;
+  }
+  for (int i = 0; i < 5; i++)
+    if (variants[i] < -1 || variants[i] > (i == 1 ? 3 : 1)) {
+      set_error("dda_op_set_variants: variant out of range (-1 .. 1; the Z pass -1 .. 3)");
+      return DDA_INVALID_ARG;
+    }
+  DDA_TRY(bind_device(op->ctx));
+  // launches already queued keep the variants they were made with
+  DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  dda_op_s::Tune& t = op->tune;
+  t.col_wide[0] = variants[0];
+  t.p3_wide = variants[1];
+  t.col_wide[1] = variants[2];
+  t.x_wide[0] = variants[3];
+  t.x_wide[1] = variants[4];
   return DDA_OK;
 }
 

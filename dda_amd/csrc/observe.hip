@@ -192,10 +192,14 @@ dda_status obs_scratch(dda_op_s* op, size_t bytes, void** out) {
 
 using namespace dda;
 
-extern "C" dda_status dda_couple_constants(int kind, uint32_t nmat, const double* m, double grid_unit, double kd,
-                                           const double prop[3], const double pol[3], double* cc, double* cc_sqrt) {
+// Alpha and sqrt(alpha) of material i, axis c, for the refractive index
+// m[(i * mstride + c * cstride)] (complex): CM or LDR (with S from prop and
+// pol, or the polarisation average).
+static dda_status couple_constants_impl(const char* fn, int kind, uint32_t nmat, const double* m, int mstride,
+                                        int cstride, double grid_unit, double kd, const double prop[3],
+                                        const double pol[3], double* cc, double* cc_sqrt) {
   if (!m || !prop || !pol || !cc || nmat == 0 || kind < 0 || kind > 2 || !(grid_unit > 0) || !(kd > 0)) {
-    set_error("dda_couple_constants: null argument, nmat 0, unknown kind, or grid_unit / kd <= 0");
+    set_error(std::string(fn) + ": null argument, nmat 0, unknown kind, or grid_unit / kd <= 0");
     return DDA_INVALID_ARG;
   }
   typedef std::complex<double> Cd;
@@ -209,25 +213,38 @@ extern "C" dda_status dda_couple_constants(int kind, uint32_t nmat, const double
     for (int a = 0; a < 3; a++) p4 += prop[a] * prop[a] * prop[a] * prop[a];
     S = 0.5 * (1 - p4);
   }
-  for (uint32_t i = 0; i < nmat; i++) {
-    const Cd mm(m[2 * i], m[2 * i + 1]), m2 = mm * mm;
-    const Cd cm = 3 * vol / (4 * kPi) * (m2 - 1.0) / (m2 + 2.0);
-    Cd a = cm;
-    if (kind != DDA_POL_CM) {
-      const Cd t1 = (b1 + (b2 + b3 * S) * m2) * kd * kd + Cd(0, 2.0 / 3.0 * kd * kd * kd);
-      a = cm / (1.0 - (cm / vol) * t1);
-    }
-    const Cd sq = std::sqrt(a);
+  for (uint32_t i = 0; i < nmat; i++)
     for (int c = 0; c < 3; c++) {
+      const double* mi = m + 2 * ((size_t)i * mstride + (size_t)c * cstride);
+      const Cd mm(mi[0], mi[1]), m2 = mm * mm;
+      const Cd cm = 3 * vol / (4 * kPi) * (m2 - 1.0) / (m2 + 2.0);
+      Cd a = cm;
+      if (kind != DDA_POL_CM) {
+        const Cd t1 = (b1 + (b2 + b3 * S) * m2) * kd * kd + Cd(0, 2.0 / 3.0 * kd * kd * kd);
+        a = cm / (1.0 - (cm / vol) * t1);
+      }
       cc[2 * (3 * i + c)] = a.real();
       cc[2 * (3 * i + c) + 1] = a.imag();
       if (cc_sqrt) {
+        const Cd sq = std::sqrt(a);
         cc_sqrt[2 * (3 * i + c)] = sq.real();
         cc_sqrt[2 * (3 * i + c) + 1] = sq.imag();
       }
     }
-  }
   return DDA_OK;
+}
+
+extern "C" dda_status dda_couple_constants(int kind, uint32_t nmat, const double* m, double grid_unit, double kd,
+                                           const double prop[3], const double pol[3], double* cc, double* cc_sqrt) {
+  return couple_constants_impl("dda_couple_constants", kind, nmat, m, 1, 0, grid_unit, kd, prop, pol, cc,
+                               cc_sqrt);
+}
+
+extern "C" dda_status dda_couple_constants_diag(int kind, uint32_t nmat, const double* m, double grid_unit,
+                                                double kd, const double prop[3], const double pol[3], double* cc,
+                                                double* cc_sqrt) {
+  return couple_constants_impl("dda_couple_constants_diag", kind, nmat, m, 3, 1, grid_unit, kd, prop, pol, cc,
+                               cc_sqrt);
 }
 
 // Observables scale by the operator's wave number: a pack qualifies only when

@@ -23,7 +23,8 @@ LDR_B1, LDR_B2, LDR_B3 = 1.8915316, -0.1648469, 1.7700004  # FPConst.hpp:48-50 (
 class DipoleGeometry:
     """Dipoles sorted by (z, y, x) with material indices and the box."""
 
-    def __init__(self, pos, mat=None, materials=None, grid_unit=1.0):
+    def __init__(self, pos, mat=None, materials=None, grid_unit=1.0, shape=None):
+        self.shape = shape  # the generating shape ("sphere", ...), None for a dipole list
         self.pos = np.ascontiguousarray(pos, np.uint32).reshape(-1, 3)
         self.nv = self.pos.shape[0]
         self.mat = np.zeros(self.nv, np.uint8) if mat is None else np.ascontiguousarray(mat, np.uint8)
@@ -53,13 +54,13 @@ def sphere(radius_units, material=1.5 + 0.1j, grid_unit=1.0):
     xd, yd, zd = x - r + 0.5, y - r + 0.5, z - r + 0.5
     keep = xd * xd + yd * yd + zd * zd <= r * r
     pos = np.stack([x[keep], y[keep], z[keep]], axis=1)
-    return DipoleGeometry(pos, None, [material], grid_unit)
+    return DipoleGeometry(pos, None, [material], grid_unit, shape="sphere")
 
 
 def box(nx, ny, nz, material=1.5 + 0.1j, grid_unit=1.0):
     """Shapes.cpp:130-143."""
     x, y, z = _zyx_grid(int(math.ceil(nx)), int(math.ceil(ny)), int(math.ceil(nz)))
-    return DipoleGeometry(np.stack([x, y, z], axis=1), None, [material], grid_unit)
+    return DipoleGeometry(np.stack([x, y, z], axis=1), None, [material], grid_unit, shape="box")
 
 
 def cylinder(length_units, radius_units, material=1.5 + 0.1j, grid_unit=1.0):
@@ -70,7 +71,8 @@ def cylinder(length_units, radius_units, material=1.5 + 0.1j, grid_unit=1.0):
     x, y, z = _zyx_grid(L, n, n)
     yd, zd = y - r + 0.5, z - r + 0.5
     keep = yd * yd + zd * zd <= r * r
-    return DipoleGeometry(np.stack([x[keep], y[keep], z[keep]], axis=1), None, [material], grid_unit)
+    return DipoleGeometry(np.stack([x[keep], y[keep], z[keep]], axis=1), None, [material], grid_unit,
+                          shape="cylinder")
 
 
 def ellipsoid(a, b, c, material=1.5 + 0.1j, grid_unit=1.0):
@@ -79,7 +81,8 @@ def ellipsoid(a, b, c, material=1.5 + 0.1j, grid_unit=1.0):
     x, y, z = _zyx_grid(nx, ny, nz)
     xd, yd, zd = (x - a + 0.5) / a, (y - b + 0.5) / b, (z - c + 0.5) / c
     keep = xd * xd + yd * yd + zd * zd <= 1.0
-    return DipoleGeometry(np.stack([x[keep], y[keep], z[keep]], axis=1), None, [material], grid_unit)
+    return DipoleGeometry(np.stack([x[keep], y[keep], z[keep]], axis=1), None, [material], grid_unit,
+                          shape="ellipsoid")
 
 
 def load_dipole_file(text, materials=None, grid_unit=1.0):
@@ -161,6 +164,26 @@ def couple_constants(params, prop, incpol, kind="ldr"):
         t1 = (LDR_B1 + (LDR_B2 + LDR_B3 * S) * m2) * kd * kd + 1j * (2.0 / 3.0) * kd ** 3
         cc = unc / (1 - (unc / vol) * t1)
     return cc, np.sqrt(cc)
+
+
+def rotation_symmetric(geometry, prop, orientation_none=True):
+    """Whether one solve (polarisation 1) covers both polarisations: the
+    reference's rule (DDA/DDAMain.cpp:105-121, 252-260) for a plane wave
+    (Beam.cpp:107-109): propagation along +z (|prop - z|^2 <= 1e-12), no
+    particle rotation, every material isotropic and a shape whose dipole model
+    is symmetric under a quarter turn about z (the sphere alone,
+    Shapes.hpp:48). Polarisation 2's solution is then polarisation 1's rotated
+    (FarFieldCalc.cpp:167-168) and its cross sections are polarisation 1's
+    (DDAMain.cpp:461-464)."""
+    p = np.asarray(prop, np.float64)
+    p = p / np.linalg.norm(p)
+    if float(np.sum((p - np.array([0.0, 0.0, 1.0])) ** 2)) > 1e-12 or not orientation_none:
+        return False
+    for m in geometry.materials:
+        v = np.broadcast_to(np.asarray(m, np.complex128), (3,))
+        if not (v[0] == v[1] and v[0] == v[2]):
+            return False
+    return geometry.shape == "sphere"
 
 
 def einc_plane(params, prop, pol):

@@ -120,4 +120,10 @@ def run_stream(server, jobs, rate=None):
             "solves_per_s": solves / (end - start), "p50_ms": 1e3 * float(np.percentile(lat, 50)),
             "p99_ms": 1e3 * float(np.percentile(lat, 99)), "mean_ms": 1e3 * float(lat.mean()),
             "packed": sum(1 for j in jobs if j.result["packed_rhs"] > len(j.result["infos"])),
-            "cache_hits": sum(j.result["cache_hit"] for j in jobs)}
+            "cache_hits": sum(j.result["cache_hit"] for j in jobs),
+            "cache_hit_rate": sum(j.result["cache_hit"] for j in jobs) / len(jobs),
+            # per job: set-up (operator, tensor, solver) and, of it, the tensor lookup or generation
+            "setup_ms_mean": 1e3 * float(np.mean([j.result["setup_s"] for j in jobs])),
+            "tensor_ms_mean": 1e3 * float(np.mean([j.result["tensor_s"] for j in jobs])),
+            "tensor_ms_p50": 1e3 * float(np.percentile([j.result["tensor_s"] for j in jobs], 50)),
+            "tensor_ms_max": 1e3 * float(np.max([j.result["tensor_s"] for j in jobs]))}

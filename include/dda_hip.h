@@ -178,13 +178,22 @@ dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs
 dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
 /* Autotuner (no counterpart in the reference, whose FFT plans are fixed per
  * length): times each variant of the column passes (plan width: 8 or 16
- * points per thread) on this operator's own buffers for nrhs right-hand sides
- * and keeps the fastest per pass. The choice is cached for the process by
- * (grid, box, precision), so later operators of that shape start tuned, and the
- * FP32 twin of a mixed-mode solver over a tuned operator is tuned too. Results
- * differ between variants only by rounding. `report` (nullable) receives a
- * one-line summary of the timings. */
+ * points per thread; for 1024-point Z columns also the 32 x 32 plan over 16 or
+ * 8 columns per work-group) on this operator's own buffers for nrhs
+ * right-hand sides and keeps the fastest per pass. The choice is cached for the
+ * process by (grid, box, precision), so later operators of that shape start
+ * tuned, and the FP32 twin of a mixed-mode solver over a tuned operator is tuned
+ * too. Results differ between variants only by rounding. `report` (nullable)
+ * receives a one-line summary of the timings. */
 dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
+/* The kernel variant of each pass, in the tuner's order: Y forward, Z with
+ * the tensor product, Y inverse, X forward, X inverse. -1 = the precision's
+ * default, 0 = narrow plan, 1 = wide plan; the Z pass of a 1024-point grid
+ * also takes 2 and 3 (the 32 x 32 plan over 16 or 8 columns). Setting pins the
+ * variants (e.g. a stored tuning, or a parity test of one variant); values a
+ * pass does not have run as its default. No counterpart in the reference. */
+dda_status dda_op_get_variants(dda_op_t op, int variants[5]);
+dda_status dda_op_set_variants(dda_op_t op, const int variants[5]);
 /* MatVecGpu::apply, DDA/MatVecGpu.cpp:51-68: host complex double in and out;
  * synchronous. */
 dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);
@@ -301,6 +310,12 @@ typedef enum dda_polarizability {
  * latter nullable) are [nmat][3] complex (m^3, m^(3/2)). Host-only. */
 dda_status dda_couple_constants(int kind, uint32_t nmat, const double* m, double grid_unit, double kd,
                                 const double prop[3], const double pol[3], double* cc, double* cc_sqrt);
+/* The same for anisotropic materials: m is [nmat][3] complex, the diagonal of
+ * each material's refractive-index tensor (the reference's
+ * DiagMatrix3<cldouble> materials, PolarizabilityDescription.cpp:69-75, 90-96:
+ * every axis uses its own m). */
+dda_status dda_couple_constants_diag(int kind, uint32_t nmat, const double* m, double grid_unit, double kd,
+                                     const double prop[3], const double pol[3], double* cc, double* cc_sqrt);
 /* Plane-wave incident fields on the device, DDA/Beam.cpp:110-118:
  * einc_dev[r][c][i] = exp(i k r_i . prop_r) pol_r[c] with r_i measured from the
  * box centre (DipoleGeometry::moveToCenter); prop, pol host [nrhs][3];

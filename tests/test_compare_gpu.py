@@ -81,3 +81,43 @@ def test_box_fixture_against_oracle(ctx, kind, mode, limit):
         assert dj < 1e-6 and dp < 1e-6
     s.close()
     op.close()
+
+
+@pytest.mark.parametrize("kind,mode,limit", [("qmr", "fp64", 1e-4), ("bicgstab", "mixed", 1e-4)])
+def test_sphere_c1_jones_grid_against_oracle(ctx, kind, mode, limit):
+    """The plain-method workload: sphere 40 dipoles across (33,552 dipoles,
+    80^3), m = 1.5 + 0.1i, lambda 6 um, d 0.5 um, incident along z, both
+    polarisations to 1e-8; the Jones matrices on the 2 deg (theta) x 6 deg (phi)
+    grid of adda_scat_params.dat and the dipole polarisations against the
+    oracle's QMR-CS solve to 1e-10, by FieldDiff's normalised RMS."""
+    print("c1 jones", kind, mode, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("sphere_c1")
+    _, pol1, pol2 = P.beam_pols(prop)
+    ccs = np.array([P.couple_constants(params, prop, p, "ldr")[1] for p in (pol1, pol2)])
+    einc = np.array([P.einc_plane(params, prop, p) for p in (pol1, pol2)])
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, grid=params.grid)
+    op.set_couple_constants_batch(ccs)
+    e_dev = D.DeviceVector(ctx, 2 * params.vec_size, 64)
+    e_dev.upload(einc.reshape(-1))
+    p_dev = D.DeviceVector(ctx, 2 * params.vec_size, 64)
+    s = D.SOLVER_CLASSES[kind](op, mode=mode, max_iter=5000)
+    infos = s.solve_device(e_dev, p_dev, 2, 1e-8)
+    assert all(i["status"] == D.OK and i["true_rel_residual"] < 1e-8 for i in infos)
+    gr = _scat_grid()
+    (nt, t0, dt), (nph, f0, df) = gr["theta"], gr["phi"]
+    J = D.jones(op, p_dev, 0, prop, pol1, pol2, nt, t0, dt, nph, f0, df)
+    p = p_dev.download().reshape(2, -1)
+    refs = []
+    for r in range(2):
+        orc = cases.oracle_problem(g, params, ccs[r])
+        st, pr, _, _ = orc.solve("qmr", einc[r], 1e-10, max_iter=5000)
+        assert st == 0
+        refs.append(pr)
+    Jr = cases.oracle_problem(g, params, ccs[0]).jones(refs[0], refs[1], 0, prop, pol1, pol2, nt, t0, dt, nph, f0, df)
+    dj = O.field_diff(J, Jr)
+    dp = max(O.field_diff(p[r], refs[r]) for r in range(2))
+    print("c1", kind, mode, "FieldDiff Jones %.2e, DipPol %.2e" % (dj, dp), flush=True)
+    assert dj < limit and dp < limit
+    assert dj < 1e-6 and dp < 1e-6  # the solves meet 1e-8
+    s.close()
+    op.close()

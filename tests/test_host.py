@@ -50,3 +50,44 @@ def test_fft_fit_static():
             1100: 1120}
     for x, n in want.items():
         assert D.lib().dda_fft_fit_static(x) == n, x
+
+
+def test_rotation_symmetric_rule():
+    """DDAMain.cpp:105-121, 252-260: one solve covers both polarisations only
+    for an isotropic sphere lit along +z without rotation."""
+    s = P.sphere(5.0, 1.5 + 0.1j, 1e-7)
+    assert P.rotation_symmetric(s, (0, 0, 1))
+    assert P.rotation_symmetric(s, (0, 0, 2.0))  # normalised first
+    assert not P.rotation_symmetric(s, (0, 0, -1))
+    assert not P.rotation_symmetric(s, (0.01, 0, 1))
+    assert not P.rotation_symmetric(s, (0, 0, 1), orientation_none=False)
+    assert not P.rotation_symmetric(P.box(6, 6, 6, 1.5, 1e-7), (0, 0, 1))
+    assert not P.rotation_symmetric(P.ellipsoid(4, 4, 4, 1.5, 1e-7), (0, 0, 1))
+    aniso = P.sphere(5.0, 1.5 + 0.1j, 1e-7)
+    aniso.materials = [np.array([1.5, 1.5, 1.6 + 0.1j])]
+    assert not P.rotation_symmetric(aniso, (0, 0, 1))
+    iso3 = P.sphere(5.0, 1.5 + 0.1j, 1e-7)
+    iso3.materials = [np.array([1.5 + 0.1j] * 3)]
+    assert P.rotation_symmetric(iso3, (0, 0, 1))
+    with open(cases.os.path.join(cases.GOLDEN, "box_10_8_6.geom")) as f:
+        assert not P.rotation_symmetric(P.load_dipole_file(f.read(), [1.33], 1e-7), (0, 0, 1))
+
+
+def test_anisotropic_couple_constants_per_axis():
+    """dda_couple_constants_diag (DiagMatrix3 materials,
+    PolarizabilityDescription.cpp:69-75, 90-96): axis c of an anisotropic
+    material equals the isotropic result for that axis's index, and the host
+    twin agrees."""
+    g, params, cc, cs, prop, pol = cases.build("two_materials")
+    mats = [np.array([1.5 + 0.1j, 1.7 + 0.02j, 2.0 + 0.3j]), 1.33 + 0.0j]
+    for kind in ("cm", "ldr", "ldr-avgpol"):
+        c, s = D.couple_constants(mats, g.grid_unit, params.kd, prop, pol, kind)
+        assert c.shape == (2, 3)
+        for ax in range(3):
+            ci, si = D.couple_constants([mats[0][ax], mats[1]], g.grid_unit, params.kd, prop, pol, kind)
+            np.testing.assert_array_equal(c[:, ax], ci[:, ax])
+            np.testing.assert_array_equal(s[:, ax], si[:, ax])
+        g2 = P.DipoleGeometry(g.pos, g.mat, mats, g.grid_unit)
+        c_ref, s_ref = P.couple_constants(P.DDAParams(g2, params.lambda_, params.grid), prop, pol, kind)
+        np.testing.assert_allclose(c, c_ref, rtol=1e-13)
+        np.testing.assert_allclose(s, s_ref, rtol=1e-13)

@@ -87,6 +87,39 @@ def test_matvec_static_plans(ctx, grid, precision):
     assert cases.rel_err(y, y_ref) <= (FP64_TOL if precision == 64 else FP32_TOL)
 
 
+@pytest.mark.parametrize("grid", [(32, 16, 1024), (48, 24, 1024)])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_matvec_z1024_variants(ctx, grid, variant):
+    """Every Z-pass variant at 1024 points (narrow and wide plans; the 32 x 32
+    plan over 16 and over 8 columns per work-group), pinned through
+    dda_op_set_variants on an FP32 operator, against the oracle; 48 columns
+    leave a partial tile for 32 and for 16 columns per work-group."""
+    print("grid", grid, "variant", variant, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("ellipsoid_5_4_3", grid=grid)
+    assert params.grid == grid
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params, seed=12)
+    y_ref = orc.matvec(x, precision=64)
+    op = _ops(ctx, g, params, cs, 32)
+    assert op.variants([-1, variant, -1, -1, -1])[1] == variant
+    for conj in (False, True):
+        y = D.MatVecGpu(op).apply(x, conj=conj)
+        y_ref = orc.matvec(x, conj=conj, precision=64)
+        assert cases.rel_err(y, y_ref) <= FP32_TOL
+    op.close()
+
+
+def test_variants_rejects_out_of_range(ctx):
+    g, params, cc, cs, prop, pol = cases.build("sphere_r4")
+    op = _ops(ctx, g, params, cs, 32)
+    with pytest.raises(D.DDAError):
+        op.variants([-1, 4, -1, -1, -1])
+    with pytest.raises(D.DDAError):
+        op.variants([2, -1, -1, -1, -1])
+    assert op.variants() == [-1, -1, -1, -1, -1]
+    op.close()
+
+
 def test_dmatrix_device_generation_matches_oracle(ctx):
     """FP64 tensor generated on the GPU vs DMatrixCpu::createDMatrix restated."""
     import oracle as O
