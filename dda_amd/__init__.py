@@ -45,6 +45,7 @@ EXPORTS = [
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
     "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_couple_constants_diag", "dda_incident_plane",
     "dda_cross_sections",
+    "dda_cross_sections_noalloc",
     "dda_far_field", "dda_jones", "dda_op_create_pack", "dda_op_pack_info", "dda_op_pack_map",
     "dda_op_set_rhs_members",
 ]
@@ -166,6 +167,7 @@ def lib():
         L.dda_couple_constants_diag.argtypes = [i32, u32, dp, dbl, dbl, dp, dp, dp, dp]
         L.dda_incident_plane.argtypes = [vp, i32, dp, dp, vp]
         L.dda_cross_sections.argtypes = [vp, vp, vp, i32, dp, dp, dp]
+        L.dda_cross_sections_noalloc.argtypes = [vp, vp, vp, i32, dp, dp, dp]
         L.dda_far_field.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int), dp, i32, dp]
         L.dda_jones.argtypes = [vp, vp, i32, dp, dp, dp, u32, dbl, dbl, u32, dbl, dbl, dp]
         L.dda_op_create_pack.argtypes = [vp, ctypes.POINTER(OpDesc), i32, i32, vp, pp]
@@ -806,6 +808,18 @@ def cross_sections(op, pol_dev, einc_dev, nrhs, cc=None):
     _check(lib().dda_cross_sections(op.handle, pol_dev.ptr, einc_dev.ptr, int(nrhs),
                                     None if c is None else _dp(c.view(np.float64)), _dp(cext),
                                     None if cabs is None else _dp(cabs)), "dda_cross_sections")
+    return cext, cabs
+
+
+def cross_sections_noalloc(op, pol_dev, einc_dev, nrhs, cc=None):
+    """cross_sections through dda_cross_sections_noalloc (no device allocation:
+    the library's fixed per-GPU scratch array; overlapping calls on one GPU race)."""
+    cext = np.zeros(nrhs)
+    cabs = np.zeros(nrhs) if cc is not None else None
+    c = None if cc is None else np.ascontiguousarray(np.asarray(cc, np.complex128).reshape(nrhs, -1, 3))
+    _check(lib().dda_cross_sections_noalloc(op.handle, pol_dev.ptr, einc_dev.ptr, int(nrhs),
+                                            None if c is None else _dp(c.view(np.float64)), _dp(cext),
+                                            None if cabs is None else _dp(cabs)), "dda_cross_sections_noalloc")
     return cext, cabs
 
 

@@ -38,6 +38,12 @@ extern "C" dda_status dda_ctx_create(int device, dda_ctx_t* out) {
     delete c;
     return DDA_HIP_ERROR;
   }
+  const dda_status rs = noalloc_scratch_setup(device);
+  if (rs != DDA_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rs;
+  }
   *out = c;
   return DDA_OK;
 }

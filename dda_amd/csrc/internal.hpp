@@ -79,6 +79,12 @@ template <typename R>
 dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj, hipStream_t st,
                          const int* done, int nrhs = 1, int done_stride = 0);
 
+// Takes the device address of the fixed scratch array of the allocation-free
+// entry point (observe.hip) for `device`, which must be current; once per
+// device, later calls return at once. dda_ctx_create calls it.
+constexpr int kMaxNoAllocDevices = 64;
+dda_status noalloc_scratch_setup(int device);
+
 }  // namespace dda
 
 struct dda_ctx_s {

@@ -10,9 +10,17 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
+
+// The fixed device scratch of dda_cross_sections_noalloc: one copy of this
+// array per GPU the library runs on, held until the process ends. Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address); it uses the per-device address that
+// noalloc_scratch_setup takes with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[DDA_NOALLOC_SCRATCH_DOUBLES];
 
 namespace dda {
 namespace {
@@ -55,24 +63,36 @@ __device__ inline double block_sum(double v) {
   return t;
 }
 
-// Per-RHS partials of sum Im(p . conj(E)) and sum multDr |p|^2.
+// Per-RHS partials of sum Im(p . conj(E)) and sum multDr |p|^2, where
+// multDr = -Im(1/alpha) - 2/3 k^3 comes from mdr [nrhs][nmat][3] or, when mdr
+// is null, from alpha itself (ccd [nrhs][nmat][3] complex, nullable: then the
+// second sum is 0).
 __global__ __launch_bounds__(kObsThreads) void k_cross(const double2* __restrict__ pol,
                                                        const double2* __restrict__ einc,
                                                        const uint8_t* __restrict__ mat,
-                                                       const double* __restrict__ mdr, int nv, int S, int nmat,
-                                                       double* part, const int* member) {
+                                                       const double* __restrict__ mdr,
+                                                       const double2* __restrict__ ccd, double k3, int nv, int S,
+                                                       int nmat, double* part, const int* member) {
   const int r = blockIdx.y;
   if (member) mat += (size_t)member[r] * nv;
   const double2* p = pol + (long long)r * 3 * S;
   const double2* e = einc + (long long)r * 3 * S;
-  const double* md = mdr + (long long)r * 3 * nmat;
+  const double* md = mdr ? mdr + (long long)r * 3 * nmat : nullptr;
+  const double2* ca = ccd ? ccd + (long long)r * 3 * nmat : nullptr;
   double se = 0, sa = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
     const int m = mat[i];
     for (int c = 0; c < 3; c++) {
       const double2 pv = p[(long long)c * S + i], ev = e[(long long)c * S + i];
       se += pv.y * ev.x - pv.x * ev.y;
-      sa += md[3 * m + c] * (pv.x * pv.x + pv.y * pv.y);
+      double w = 0;
+      if (md) {
+        w = md[3 * m + c];
+      } else if (ca) {
+        const double2 a = ca[3 * m + c];
+        w = a.y / (a.x * a.x + a.y * a.y) - k3;  // -Im(1/a) - 2/3 k^3
+      }
+      sa += w * (pv.x * pv.x + pv.y * pv.y);
     }
   }
   const double te = block_sum(se), ta = block_sum(sa);
@@ -80,6 +100,20 @@ __global__ __launch_bounds__(kObsThreads) void k_cross(const double2* __restrict
     part[((long long)r * gridDim.x + blockIdx.x) * 2] = te;
     part[((long long)r * gridDim.x + blockIdx.x) * 2 + 1] = ta;
   }
+}
+
+// Second stage of k_cross, one thread per right-hand side, blocks summed in
+// order: out[r] = scale * sum_b part[r][b][0], out[nrhs + r] = scale * sum_b part[r][b][1].
+__global__ void k_cross_final(const double* __restrict__ part, int nb, int nrhs, double scale, double* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs) return;
+  double se = 0, sa = 0;
+  for (int b = 0; b < nb; b++) {
+    se += part[((long long)r * nb + b) * 2];
+    sa += part[((long long)r * nb + b) * 2 + 1];
+  }
+  out[r] = scale * se;
+  out[nrhs + r] = scale * sa;
 }
 
 // Far-field sums S_q = sum_j p_{rhs_q, j} exp(-i kd n_q . pos_j), first stage:
@@ -188,6 +222,35 @@ dda_status obs_scratch(dda_op_s* op, size_t bytes, void** out) {
 }
 
 }  // namespace
+
+// Device addresses of `scratch`, one per device, taken once per device with
+// that device current (dda_ctx_create calls this after hipSetDevice).
+static std::mutex g_noalloc_mu;
+static double* g_noalloc_ptr[kMaxNoAllocDevices] = {};
+
+dda_status noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxNoAllocDevices) {
+    set_error("noalloc scratch: device index out of range");
+    return DDA_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return DDA_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return DDA_HIP_ERROR;
+  }
+  g_noalloc_ptr[device] = static_cast<double*>(p);
+  return DDA_OK;
+}
+
+static double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxNoAllocDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
+
 }  // namespace dda
 
 using namespace dda;
@@ -270,8 +333,8 @@ extern "C" dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* pr
   }
   DDA_TRY(bind_device(op->ctx));
   DDA_TRY(positions_on_device(op));
-  void* scratch = nullptr;
-  DDA_TRY(obs_scratch(op, 6ull * nrhs * sizeof(double), &scratch));
+  void* buf = nullptr;
+  DDA_TRY(obs_scratch(op, 6ull * nrhs * sizeof(double), &buf));
   std::vector<double> h(6ull * nrhs);
   for (int r = 0; r < nrhs; r++)
     for (int a = 0; a < 3; a++) {
@@ -279,10 +342,10 @@ extern "C" dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* pr
       h[6 * r + 3 + a] = pol[3 * r + a];
     }
   hipStream_t st = op->ctx->stream;
-  DDA_TRY_HIP(hipMemcpyAsync(scratch, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DDA_TRY_HIP(hipMemcpyAsync(buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
   const MvDims& d = op->dims;
   hipLaunchKernelGGL(k_incident, dim3(std::min(1024, (d.S + 255) / 256), nrhs), dim3(256), 0, st, (double2*)einc_dev,
-                     (const uint32_t*)op->pos_dev, (const double*)scratch, d.nv, d.S, op->wave_num, frame_of(op));
+                     (const uint32_t*)op->pos_dev, (const double*)buf, d.nv, d.S, op->wave_num, frame_of(op));
   DDA_TRY_HIP(hipGetLastError());
   DDA_TRY_HIP(hipStreamSynchronize(st));
   return DDA_OK;
@@ -305,16 +368,16 @@ extern "C" dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const
       const std::complex<double> a(cc[2 * i], cc[2 * i + 1]);
       mdr[i] = -(1.0 / a).imag() - 2.0 / 3.0 * k * k * k;
     }
-  void* scratch = nullptr;
+  void* buf = nullptr;
   const size_t pbytes = 2ull * nb * nrhs * sizeof(double);
-  DDA_TRY(obs_scratch(op, pbytes + mdr.size() * sizeof(double), &scratch));
-  double* part = (double*)scratch;
+  DDA_TRY(obs_scratch(op, pbytes + mdr.size() * sizeof(double), &buf));
+  double* part = (double*)buf;
   double* mdr_dev = part + 2ull * nb * nrhs;
   hipStream_t st = op->ctx->stream;
   DDA_TRY_HIP(hipMemcpyAsync(mdr_dev, mdr.data(), mdr.size() * sizeof(double), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_cross, dim3(nb, nrhs), dim3(kObsThreads), 0, st, (const double2*)pol_dev,
-                     (const double2*)(einc_dev ? einc_dev : pol_dev), (const uint8_t*)op->mat_dev, mdr_dev, d.nv, d.S,
-                     (int)op->nmat, part, d.member);
+                     (const double2*)(einc_dev ? einc_dev : pol_dev), (const uint8_t*)op->mat_dev, mdr_dev,
+                     (const double2*)nullptr, 0.0, d.nv, d.S, (int)op->nmat, part, d.member);
   DDA_TRY_HIP(hipGetLastError());
   std::vector<double> h(2ull * nb * nrhs);
   DDA_TRY_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -328,6 +391,54 @@ extern "C" dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const
     if (cext) cext[r] = 4 * kPi * k * se;
     if (cabs) cabs[r] = 4 * kPi * k * sa;
   }
+  return DDA_OK;
+}
+
+// dda_cross_sections without allocating: alpha, the partial sums and the
+// finished sums go through the device's fixed `scratch` array (its address
+// from noalloc_scratch_setup), the per-right-hand-side sums are finished on the
+// device and copied straight into cext / cabs. No hipMalloc, hipFree or host
+// heap allocation happens here. Every k_cross block writes its own partials,
+// so nothing needs clearing between calls. Calls that overlap on one GPU share
+// the array and race.
+extern "C" dda_status dda_cross_sections_noalloc(dda_op_t op, const void* pol_dev, const void* einc_dev, int nrhs,
+                                                 const double* cc, double* cext, double* cabs) {
+  if (!op || !pol_dev || nrhs < 1 || (cext && !einc_dev) || (cabs && !cc)) {
+    set_error("dda_cross_sections_noalloc: null argument or nrhs < 1");
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY(one_wave_number(op, "dda_cross_sections_noalloc"));
+  const MvDims& d = op->dims;
+  const int nb = std::max(1, std::min(256, (d.nv + 4095) / 4096));
+  const size_t ncc = cc ? 6ull * op->nmat * nrhs : 0;  // alpha, [nrhs][nmat][3] complex
+  const size_t npart = 2ull * nb * nrhs, nout = 2ull * nrhs;
+  if (ncc + npart + nout > (size_t)DDA_NOALLOC_SCRATCH_DOUBLES) {
+    set_error("dda_cross_sections_noalloc: needs " + std::to_string(ncc + npart + nout) +
+              " doubles of scratch (6 nmat nrhs + 2 nrhs (blocks + 1)), above the fixed " +
+              std::to_string(DDA_NOALLOC_SCRATCH_DOUBLES));
+    return DDA_INVALID_ARG;
+  }
+  DDA_TRY(bind_device(op->ctx));
+  double* base = noalloc_scratch(op->ctx->device);
+  if (!base) {
+    set_error("dda_cross_sections_noalloc: the device scratch of this context's device was not set up");
+    return DDA_HIP_ERROR;
+  }
+  double* ccd = base;
+  double* part = base + ncc;
+  double* fin = part + npart;
+  const double k = op->wave_num;
+  hipStream_t st = op->ctx->stream;
+  if (cc) DDA_TRY_HIP(hipMemcpyAsync(ccd, cc, ncc * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_cross, dim3(nb, nrhs), dim3(kObsThreads), 0, st, (const double2*)pol_dev,
+                     (const double2*)(einc_dev ? einc_dev : pol_dev), (const uint8_t*)op->mat_dev,
+                     (const double*)nullptr, cc ? (const double2*)ccd : (const double2*)nullptr,
+                     2.0 / 3.0 * k * k * k, d.nv, d.S, (int)op->nmat, part, d.member);
+  hipLaunchKernelGGL(k_cross_final, dim3((nrhs + 255) / 256), dim3(256), 0, st, part, nb, nrhs, 4 * kPi * k, fin);
+  DDA_TRY_HIP(hipGetLastError());
+  if (cext) DDA_TRY_HIP(hipMemcpyAsync(cext, fin, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (cabs) DDA_TRY_HIP(hipMemcpyAsync(cabs, fin + nrhs, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+  DDA_TRY_HIP(hipStreamSynchronize(st));
   return DDA_OK;
 }
 
@@ -350,9 +461,9 @@ extern "C" dda_status dda_far_field(dda_op_t op, const void* pol_dev, int nrhs, 
   const int nchunk = (d.nv + chunk - 1) / chunk;
   const size_t qbytes = 4ull * nq * sizeof(double), pbytes = 6ull * nchunk * nq * sizeof(double);
   const size_t obytes = 6ull * nq * sizeof(double);
-  void* scratch = nullptr;
-  DDA_TRY(obs_scratch(op, qbytes + pbytes + obytes, &scratch));
-  double* qd = (double*)scratch;
+  void* buf = nullptr;
+  DDA_TRY(obs_scratch(op, qbytes + pbytes + obytes, &buf));
+  double* qd = (double*)buf;
   double* part = qd + 4ull * nq;
   double* od = part + 6ull * nchunk * nq;
   std::vector<double> hq(4ull * nq);

@@ -328,6 +328,19 @@ dda_status dda_incident_plane(dda_op_t op, int nrhs, const double* prop, const d
  * [nrhs], either may be NULL. */
 dda_status dda_cross_sections(dda_op_t op, const void* pol_dev, const void* einc_dev, int nrhs, const double* cc,
                               double* cext, double* cabs);
+/* dda_cross_sections without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device (or host heap) memory — serving
+ * loops and harnesses that forbid hipMalloc. Its partial sums live in a fixed
+ * device array of DDA_NOALLOC_SCRATCH_DOUBLES doubles per GPU (8 MiB, held
+ * until the process ends), which needs 6 nmat nrhs (with cc) + 2 nrhs (blocks
+ * + 1) doubles, blocks = clamp(ceil(nv / 4096), 1, 256); larger calls fail
+ * with DDA_INVALID_ARG. Same arguments and results as dda_cross_sections (sums
+ * in the same order; Cabs's -Im(1/alpha) is formed on the device, so it may
+ * differ in the last bits). Calls that overlap on one GPU share the array and
+ * race: serialise them. */
+#define DDA_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+dda_status dda_cross_sections_noalloc(dda_op_t op, const void* pol_dev, const void* einc_dev, int nrhs,
+                                      const double* cc, double* cext, double* cabs);
 /* Scattered far fields, DDA/CpuFieldCalculator.cpp:48-98: for query q, the
  * field of right-hand side rhs[q] (NULL = all 0) in unit direction dirs[q]
  * (host [nq][3]); out host [nq][3] complex. All queries run in one two-stage
