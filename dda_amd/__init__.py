@@ -392,8 +392,9 @@ class GpuMatVec:
         return buf.value.decode()
 
     def variants(self, values=None):
-        """Kernel variant per pass (Y fwd, Z, Y inv, X fwd, X inv); with `values`, pins them first."""
-        a = (ctypes.c_int32 * 5)()
+        """Kernel variant per pass (Y fwd, Z, Y inv, X fwd, X inv) and the two-stream
+        block split; with `values`, pins them first."""
+        a = (ctypes.c_int32 * 6)()
         if values is not None:
             a[:] = [int(v) for v in values]
             _check(lib().dda_op_set_variants(self.handle, a), "dda_op_set_variants")

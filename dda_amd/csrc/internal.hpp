@@ -153,6 +153,7 @@ struct dda_op_s {
     int col_wide[2] = {-1, -1};  // forward / inverse Y pass
     int p3_wide = -1;            // Z pass with the tensor product
     int x_wide[2] = {-1, -1};    // X passes (scatter + forward, inverse + gather)
+    int split = -1;              // a block as two half-blocks on two streams: -1 default (off), 0 off, 1 on
   } tune;
   bool tuned = false;
   struct dda_dhat_cache_s* dhat_cache = nullptr;  // set when dhat is a cache entry (not owned)
@@ -179,7 +180,7 @@ struct dda_op_s {
   // A block of right-hand sides runs as two halves on two streams (the Z pass
   // of one beside the X and Y passes of the other); the second stream and
   // the fork / join events are made on first use.
-  bool split_rhs = true;
+  bool split_rhs = false;  // the default of tune.split (DDA_SPLIT=1 turns it on)
   hipStream_t split_stream = nullptr;
   hipEvent_t split_ev[2] = {nullptr, nullptr};
 };

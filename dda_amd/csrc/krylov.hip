@@ -902,7 +902,7 @@ struct Core {
     std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat,
                                   reinterpret_cast<const void*>((intptr_t)nrhs)};
     for (int t : {op->tune.col_wide[0], op->tune.col_wide[1], op->tune.p3_wide, op->tune.x_wide[0],
-                  op->tune.x_wide[1]})
+                  op->tune.x_wide[1], op->tune.split, (int)op->split_rhs})
       g.push_back(reinterpret_cast<const void*>((intptr_t)t));
     for (C* p : v) g.push_back(p);
     return g;
@@ -1603,6 +1603,8 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
     DDA_TRY(dda_op_set_cc_sqrt_batch(s->op32, op->cc_sqrt_host.data(), nrhs));
     if (op->pack.nmember) DDA_TRY(dda_op_set_rhs_members(s->op32, op->pack.member_host.data(), nrhs));
     s->op32->prof = op->prof;
+    s->op32->tune.split = op->tune.split;  // and its block split choice
+    s->op32->split_rhs = op->split_rhs;
   }
   const size_t bytes = (size_t)nrhs * s->n * sizeof(double2);
   const hipMemcpyKind in_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;

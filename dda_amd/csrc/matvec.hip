@@ -974,12 +974,14 @@ dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, 
   return launch_pass3_plan<R, N, StaticPlan<N, W>, T>(op, v, st, done, nrhs);
 }
 
-// Z-pass variants (Tune::p3_wide): -1 the precision's default plan width,
-// 0 narrow, 1 wide; at 1024 points also 2 and 3, the 32 x 32 plan (32 points
-// per thread, one LDS exchange per transform, 2 waves per SIMD, FP32) over 16
+// Z-pass variants (Tune::p3_wide): -1 the precision's default, 0 narrow,
+// 1 wide; at 1024 points also 2 and 3, the 32 x 32 plan (32 points per
+// thread, one LDS exchange per transform, 2 waves per SIMD, FP32) over 16
 // columns (512 threads, 128-byte segments, one work-group per CU by LDS) or
 // 8 columns (256 threads, 64-byte segments, two work-groups per CU).
-// FP32 only: in FP64 the 32-point plan's data alone needs 384 VGPRs.
+// FP32 only: in FP64 the 32-point plan's data alone needs 384 VGPRs. The FP32
+// default at 1024 points is 2: measured at 1024^3 (sphere 512 across) 14.8 ms
+// against 17.8 ms (wide), 19.2 ms (variant 3) and 33.0 ms (narrow).
 typedef PlanOf<1024, 32, RadixList<32, 32>> Plan1024x32;
 inline int p3_variants(int n, int prec) { return n == 1024 && prec == 32 ? 4 : 2; }
 
@@ -987,7 +989,7 @@ template <typename R, int N>
 dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
   const int var = op->tune.p3_wide;
   if constexpr (N == 1024 && sizeof(R) == 4) {
-    if (var == 2) return launch_pass3_plan<R, N, Plan1024x32, 16>(op, v, st, done, nrhs);
+    if (var == 2 || var < 0) return launch_pass3_plan<R, N, Plan1024x32, 16>(op, v, st, done, nrhs);
     if (var == 3) return launch_pass3_plan<R, N, Plan1024x32, 8>(op, v, st, done, nrhs);
   }
   return plan_wide<R>(var) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
@@ -1289,9 +1291,11 @@ static dda_status split_resources(dda_op_s* op) {
 // A block of two or more right-hand sides with the split enabled runs as two
 // halves on two streams, staggered by two passes: the second half starts once
 // the first has finished its Y pass, so the Z pass of each half (bound by its
-// arithmetic) runs beside the memory-bound X and Y passes of the other. The
+// arithmetic) can run beside the memory-bound X and Y passes of the other. The
 // fork and the join are event waits, which a stream capture records as graph
-// edges. Per-pass profiling keeps the block on one stream.
+// edges. Per-pass profiling keeps the block on one stream. Off unless the
+// tuner (or dda_op_set_variants) picks it: measured on the 2-RHS FP32 block
+// at 256^3, 707 us split against 642 us on one stream.
 template <typename R>
 dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done_ptr,
                          int nrhs, int done_stride) {
@@ -1307,7 +1311,7 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   done.done = done_ptr;
   done.stride = done_stride;
   done.per = 1;
-  const bool split = op->split_rhs && nrhs >= 2 && !op->prof;
+  const bool split = (op->tune.split < 0 ? op->split_rhs : op->tune.split != 0) && nrhs >= 2 && !op->prof;
   if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4);
   DDA_TRY(split_resources(op));
   const int h = (nrhs + 1) / 2;
@@ -1747,7 +1751,7 @@ static dda_status create_op(dda_ctx_t ctx, const dda_op_desc* desc, int precisio
   const size_t csz = precision == 64 ? sizeof(double2) : sizeof(float2);
   op->static_z = static_z_fits(d.Gx, d.Gy, d.Gz, csz) && !std::getenv("DDA_GENERIC_FFT");
   op->static_x = has_static_plan(d.Gx) && !std::getenv("DDA_GENERIC_FFT");
-  op->split_rhs = !std::getenv("DDA_NO_SPLIT");
+  op->split_rhs = std::getenv("DDA_SPLIT") && std::atoi(std::getenv("DDA_SPLIT")) != 0;
 
   // Launch shapes: X passes hold ~2048 points per work-group; column passes
   // take 128-byte row segments (8 complex doubles, 16 complex floats).
@@ -2477,6 +2481,33 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
     (void)hipFree(xv);
   }
   if (report->empty()) *report = "no static-plan passes to tune at this grid";
+  // a block of right-hand sides: the whole matvec on one stream or split in
+  // two halves on two streams (on the grid buffers as input and output)
+  if (nrhs >= 2 && !op->prof) {
+    void* xy = nullptr;
+    DDA_TRY_HIP(hipMalloc(&xy, 2 * 3ull * d.S * csz * nrhs));
+    hipError_t e = hipMemsetAsync(xy, 0, 2 * 3ull * d.S * csz * nrhs, st);
+    double ts[2] = {0, 0};
+    dda_status rc = e == hipSuccess ? DDA_OK : DDA_HIP_ERROR;
+    for (int sp = 0; sp < 2 && rc == DDA_OK; sp++) {
+      op->tune.split = sp;
+      char* b = (char*)xy;
+      rc = time_pass(st, reps, [&]() {
+        return launch_matvec<R>(op, b, b + 3ull * d.S * csz * nrhs, 0, st, nullptr, nrhs, 0);
+      }, &ts[sp]);
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(xy);
+    if (rc != DDA_OK) {
+      op->tune = best;
+      return rc;
+    }
+    best.split = ts[1] < ts[0] ? 1 : 0;
+    std::snprintf(buf, sizeof buf, "; block of %d: one stream %.1f us, two streams %.1f us -> %s", nrhs, ts[0],
+                  ts[1], best.split ? "two" : "one");
+    *report += buf;
+    op->tune = best;
+  }
   return DDA_OK;
 }
 
@@ -2511,23 +2542,23 @@ extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t re
   return DDA_OK;
 }
 
-extern "C" dda_status dda_op_get_variants(dda_op_t op, int variants[5]) {
+extern "C" dda_status dda_op_get_variants(dda_op_t op, int variants[6]) {
   if (!op || !variants) {
     set_error("dda_op_get_variants: null argument");
     return DDA_INVALID_ARG;
   }
   const dda_op_s::Tune& t = op->tune;
-  const int v[5] = {t.col_wide[0], t.p3_wide, t.col_wide[1], t.x_wide[0], t.x_wide[1]};
-  for (int i = 0; i < 5; i++) variants[i] = v[i];
+  const int v[6] = {t.col_wide[0], t.p3_wide, t.col_wide[1], t.x_wide[0], t.x_wide[1], t.split};
+  for (int i = 0; i < 6; i++) variants[i] = v[i];
   return DDA_OK;
 }
 
-extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[5]) {
+extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[6]) {
   if (!op || !variants) {
     set_error("dda_op_set_variants: null argument");
     return DDA_INVALID_ARG;
   }
-  for (int i = 0; i < 5; i++)
+  for (int i = 0; i < 6; i++)
     if (variants[i] < -1 || variants[i] > (i == 1 ? 3 : 1)) {
       set_error("dda_op_set_variants: variant out of range (-1 .. 1; the Z pass -1 .. 3)");
       return DDA_INVALID_ARG;
@@ -2541,6 +2572,7 @@ extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[5]) {
   t.col_wide[1] = variants[2];
   t.x_wide[0] = variants[3];
   t.x_wide[1] = variants[4];
+  t.split = variants[5];
   return DDA_OK;
 }
 

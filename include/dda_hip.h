@@ -187,13 +187,15 @@ dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int n
  * receives a one-line summary of the timings. */
 dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
 /* The kernel variant of each pass, in the tuner's order: Y forward, Z with
- * the tensor product, Y inverse, X forward, X inverse. -1 = the precision's
- * default, 0 = narrow plan, 1 = wide plan; the Z pass of a 1024-point grid
- * also takes 2 and 3 (the 32 x 32 plan over 16 or 8 columns). Setting pins the
- * variants (e.g. a stored tuning, or a parity test of one variant); values a
- * pass does not have run as its default. No counterpart in the reference. */
-dda_status dda_op_get_variants(dda_op_t op, int variants[5]);
-dda_status dda_op_set_variants(dda_op_t op, const int variants[5]);
+ * the tensor product, Y inverse, X forward, X inverse; -1 = the precision's
+ * default, 0 = narrow plan, 1 = wide plan; the Z pass of a 1024-point FP32
+ * grid also takes 2 and 3 (the 32 x 32 plan over 16 or 8 columns). Entry 5:
+ * a block of right-hand sides as two half-blocks on two streams (-1 default
+ * = off, 0 off, 1 on). Setting pins the variants (e.g. a stored tuning, or a
+ * parity test of one variant); values a pass does not have run as its
+ * default. No counterpart in the reference. */
+dda_status dda_op_get_variants(dda_op_t op, int variants[6]);
+dda_status dda_op_set_variants(dda_op_t op, const int variants[6]);
 /* MatVecGpu::apply, DDA/MatVecGpu.cpp:51-68: host complex double in and out;
  * synchronous. */
 dda_status dda_op_apply_host(dda_op_t op, const double* x, double* y, int conj);

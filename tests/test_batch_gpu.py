@@ -177,7 +177,7 @@ def test_batched_start_vectors(ctx, mixed):
 
 
 @pytest.mark.parametrize("precision,mode", [(64, "fp64"), (64, "mixed")])
-def test_split_block_matches_one_stream(ctx, precision, mode, monkeypatch):
+def test_split_block_matches_one_stream(ctx, precision, mode):
     """A block of right-hand sides runs as two halves on two streams (the Z
     pass of one beside the X and Y passes of the other); the block matvec and
     a block solve (whose steady iterations replay a captured graph holding the
@@ -186,10 +186,10 @@ def test_split_block_matches_one_stream(ctx, precision, mode, monkeypatch):
     print("split block", mode, flush=True)
     g, params, einc, ccs = _setup("two_materials", 3, 17)  # 6 right-hand sides
     k, n = len(ccs), params.vec_size
-    monkeypatch.setenv("DDA_NO_SPLIT", "1")
     op1 = _op(ctx, g, params, precision)
-    monkeypatch.delenv("DDA_NO_SPLIT")
     op2 = _op(ctx, g, params, precision)
+    assert op1.variants()[5] == -1  # off by default
+    op2.variants([-1, -1, -1, -1, -1, 1])
     rng = np.random.default_rng(8)
     x = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex128)
     for r in range(k):

@@ -101,7 +101,7 @@ def test_matvec_z1024_variants(ctx, grid, variant):
     x = P.random_dipole_vector(params, seed=12)
     y_ref = orc.matvec(x, precision=64)
     op = _ops(ctx, g, params, cs, 32)
-    assert op.variants([-1, variant, -1, -1, -1])[1] == variant
+    assert op.variants([-1, variant, -1, -1, -1, -1])[1] == variant
     for conj in (False, True):
         y = D.MatVecGpu(op).apply(x, conj=conj)
         y_ref = orc.matvec(x, conj=conj, precision=64)
@@ -113,10 +113,12 @@ def test_variants_rejects_out_of_range(ctx):
     g, params, cc, cs, prop, pol = cases.build("sphere_r4")
     op = _ops(ctx, g, params, cs, 32)
     with pytest.raises(D.DDAError):
-        op.variants([-1, 4, -1, -1, -1])
+        op.variants([-1, 4, -1, -1, -1, -1])
     with pytest.raises(D.DDAError):
-        op.variants([2, -1, -1, -1, -1])
-    assert op.variants() == [-1, -1, -1, -1, -1]
+        op.variants([2, -1, -1, -1, -1, -1])
+    with pytest.raises(D.DDAError):
+        op.variants([-1, -1, -1, -1, -1, 2])
+    assert op.variants() == [-1, -1, -1, -1, -1, -1]
     op.close()
 
 
