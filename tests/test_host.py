@@ -1,5 +1,6 @@
 """Host-side pieces of the library that run without a GPU."""
 import math
+import os
 
 import numpy as np
 
@@ -91,3 +92,59 @@ def test_anisotropic_couple_constants_per_axis():
         c_ref, s_ref = P.couple_constants(P.DDAParams(g2, params.lambda_, params.grid), prop, pol, kind)
         np.testing.assert_allclose(c, c_ref, rtol=1e-13)
         np.testing.assert_allclose(s, s_ref, rtol=1e-13)
+
+
+def test_hdf5_dippol_and_jones_files(tmp_path):
+    """DipPol and Jones files in the reference's MATLAB 7.3 layout
+    (HDF5::matlabSerialize, EMSim/DataFilesDDA.cpp:58-63,
+    EMSim/JonesCalculus.hpp:82-93): values read back bit for bit, the "MAT"
+    user block, and -- through the HDF5 tools' own reader, h5dump -- the
+    struct groups, classes, string encoding, dims and the Fortran-order Jones
+    array."""
+    import shutil
+    import subprocess
+    from dda_amd import hdf5 as H
+    if not H.available():
+        import pytest
+        pytest.skip("no HDF5 C library")
+    g, params, cc, cs, prop, pol = cases.build("two_materials")
+    _, p1, p2 = P.beam_pols(prop)
+    pv = P.random_dipole_vector(params, seed=4)
+    f1 = tmp_path / "DipPol-Pol1.hdf5"
+    H.write_dippol(f1, params, pv, 1, prop, [p1, p2], geometry_string="sphere", cmdline="dda")
+    assert np.array_equal(H.read_dippol(f1, params), pv)
+    head = open(f1, "rb").read(512)
+    assert head[:4] == b"MAT\x00" and head[125:128] == b"\x02IM"
+    assert np.array_equal(H.read_dataset(f1, "/Geometry/DipolePositions", dtype=np.uint32), g.pos)
+    assert np.array_equal(H.read_dataset(f1, "/Geometry/Size", dtype=np.uint32), np.array(g.box))
+    np.testing.assert_allclose(H.read_dataset(f1, "/Parameters/MethodParameters/Frequency"),
+                               [299792458.0 / params.lambda_], rtol=0)
+    rng = np.random.default_rng(5)
+    n = 7
+    J = rng.normal(size=(n, 2, 2)) + 1j * rng.normal(size=(n, 2, 2))
+    th, ph = np.linspace(0, 1, n), np.linspace(0, 2, n)
+    f2 = tmp_path / "Far.hdf5"
+    H.write_jones(f2, params, th, ph, J, prop, [p1, p2])
+    d = H.read_dataset(f2, "/JonesFarField/Data", complex_=True)
+    assert d.shape == (1, n, 2, 2)
+    for k in range(n):
+        for i in range(2):
+            for j in range(2):
+                assert d[0, k, j, i] == J[k, i, j]  # Data[i][j][k][0], Fortran order
+    np.testing.assert_array_equal(H.read_dataset(f2, "/JonesFarField/Theta"), th)
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if not os.path.exists(h5dump):
+        return
+    txt = subprocess.run([h5dump, "-A", str(f1)], capture_output=True, text=True, check=True).stdout
+    assert 'GROUP "Field"' in txt and 'GROUP "Geometry"' in txt and 'GROUP "Parameters"' in txt
+    assert '(0): "struct"' in txt and '(0): "char"' in txt and '(0): "uint32"' in txt
+    assert 'DATASET "Periodicity"' in txt and "SIMPLE { ( 0, 3 ) / ( 0, 3 ) }" in txt
+    assert 'H5T_IEEE_F64LE "real";' in txt and 'H5T_IEEE_F64LE "imag";' in txt
+    assert "SIMPLE { ( %d, 3 ) / ( %d, 3 ) }" % (params.nv, params.nv) in txt
+    t2 = subprocess.run([h5dump, "-A", str(f2)], capture_output=True, text=True, check=True).stdout
+    assert 'DATASET "DipolePositions"' not in t2  # far-field files carry the geometry without positions
+    assert "SIMPLE { ( 1, %d, 2, 2 ) / ( 1, %d, 2, 2 ) }" % (n, n) in t2
+    # the Type string as UTF-16 code units, dims (len, 1)
+    t3 = subprocess.run([h5dump, "-d", "/Type", str(f2)], capture_output=True, text=True, check=True).stdout
+    assert "SIMPLE { ( 13, 1 ) / ( 13, 1 ) }" in t3
+    assert all("(%d,0): %d" % (i, ord(c)) in t3 for i, c in enumerate("JonesFarField"))

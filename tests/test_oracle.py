@@ -550,3 +550,21 @@ def test_solver_stagnation_status(kind, mri, it_expected):
     # the same solve with the default limit goes on (no stagnation that early)
     st2, _, it2, _ = P.solve(kind, e, 1e-10, max_iter=it + 5)
     assert st2 != O.ST_STAGNATION
+
+
+@pytest.mark.parametrize("name", ["two_materials", "box_10_8_6", "cylinder_9_3"])
+def test_pocketfft_matvec_equals_oracle_matvec(name):
+    """The CPU baseline's MatVecCpu restatement over scipy.fft (pocketfft, the
+    stand-in for the reference's FFTW) gives the oracle's matvec to rounding,
+    both conj flags, serial and threaded."""
+    import cases
+    from dda_amd import problem as P
+    g, params, cc, cs, prop, pol = cases.build(name)
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params, seed=5)
+    for conj in (False, True):
+        y0 = orc.matvec(x, conj=conj)
+        for w in (1, 3):
+            y1 = orc.matvec_pocketfft(x, conj=conj, workers=w)
+            assert np.linalg.norm(y1 - y0) <= 1e-14 * np.linalg.norm(y0)
+            assert np.all(y1.reshape(3, -1)[:, params.nv:] == 0)
