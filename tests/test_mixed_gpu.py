@@ -57,7 +57,9 @@ def test_mixed_c1_to_1e8(ctx):
 def test_mixed_c2_block_cext(ctx):
     """Sphere 128 dipoles across, 256^3 grid: both polarisations as one 2-RHS
     block, mixed BiCGStab to 1e-6; Cext and Cabs within 1e-5 of FP64 QMR-CS
-    solved to 1e-8."""
+    solved to 1e-8 on the GPU and of the CPU oracle's FP64 QMR-CS solve to
+    1e-8 (tests/golden/c2_sphere128_fp64.json, made by make_c2_fixture.py);
+    the GPU's FP64 solve within 1e-7 of the oracle's."""
     g, params, cc1, cs1, prop, pol1 = cases.build("sphere_c2")
     _, _, cc2, cs2, _, pol2 = cases.build("sphere_c2", pol_index=2)
     einc = np.array([P.einc_plane(params, prop, pol1), P.einc_plane(params, prop, pol2)])
@@ -68,8 +70,21 @@ def test_mixed_c2_block_cext(ctx):
     print("c2 mixed bicgstab", [i["iterations"] for i in i64], flush=True)
     p, infos = D.GpuBicgStab(op, mixed=True).get_pol_vecs(einc, 1e-6)
     print([(i["iterations"], i["true_rel_residual"]) for i in infos], flush=True)
+    import json
+    import os
+    with open(os.path.join(cases.GOLDEN, "c2_sphere128_fp64.json")) as f:
+        fixture = json.load(f)
+    assert fixture["nv"] == params.nv and tuple(fixture["grid"]) == tuple(params.grid)
     for r, cc in enumerate((cc1, cc2)):
         assert i64[r]["status"] == 0 and infos[r]["status"] == 0
+        ref = fixture["pols"][r]
+        c_ref, a_ref = ref["cext"], ref["cabs"]
+        c64, a64 = P.ext_cross(params, p64[r], einc[r]), P.abs_cross(params, p64[r], cc)
+        cm, am = P.ext_cross(params, p[r], einc[r]), P.abs_cross(params, p[r], cc)
+        print("c2 pol %d vs CPU oracle: fp64 qmr cext %.2e cabs %.2e; mixed cext %.2e cabs %.2e" % (
+            r + 1, abs(c64 / c_ref - 1), abs(a64 / a_ref - 1), abs(cm / c_ref - 1), abs(am / a_ref - 1)), flush=True)
+        assert abs(c64 - c_ref) <= 1e-7 * abs(c_ref) and abs(a64 - a_ref) <= 1e-7 * abs(a_ref)
+        assert abs(cm - c_ref) <= 1e-5 * abs(c_ref) and abs(am - a_ref) <= 1e-5 * abs(a_ref)
         assert infos[r]["true_rel_residual"] < 1e-6 and infos[r]["refinements"] >= 1
         c, c64 = P.ext_cross(params, p[r], einc[r]), P.ext_cross(params, p64[r], einc[r])
         assert abs(c - c64) <= 1e-5 * abs(c64)
