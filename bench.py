@@ -530,6 +530,20 @@ def main():
                            "refinements": [x["refinements"] for x in o],
                            "true_rel_residual": [x["true_rel_residual"] for x in o],
                            "cext": [x["cext"] for x in o], "cabs": [x["cabs"] for x in o]}
+    # the CPU oracle's FP64 QMR-CS answer at this workload (to 1e-8), written by
+    # tests/golden/make_c2_fixture.py at grid unit 0.5 um; cross sections
+    # scale with d^2 at fixed kd and m
+    fixture = os.path.join(ROOT, "tests", "golden", "c2_sphere128_fp64.json")
+    if args.diameter == 128 and os.path.exists(fixture):
+        with open(fixture) as f:
+            fx = json.load(f)
+        scale = (d / 0.5e-6) ** 2
+        ref = [(p["cext"] * scale, p["cabs"] * scale) for p in fx["pols"]]
+        for m_ in modes_out.values():
+            m_["vs_cpu_oracle"] = {
+                "cext_rel_err": max(abs(a / r[0] - 1) for a, r in zip(m_["cext"], ref)),
+                "cabs_rel_err": max(abs(a / r[1] - 1) for a, r in zip(m_["cabs"], ref)),
+                "reference": "CPU oracle FP64 QMR-CS to 1e-8 (tests/golden/c2_sphere128_fp64.json)"}
     if "fp64:qmr" in modes_out:
         f = modes_out["fp64:qmr"]
         for k in list(modes_out):
@@ -590,6 +604,34 @@ def main():
                              % (reps, threads, t_mv, t_serial, threads, t_orc, model, ncpu, mv_per_solve)}
         except Exception as e:
             cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": "failed: %s" % e}
+
+    # the reference's single-solve path: a sphere lit along z is rotation
+    # symmetric (DDAMain.cpp:105-121, 252-260), so one solve (polarisation 1)
+    # gives both polarisations' results (DDAMain.cpp:380-410, 461-464)
+    symmetric = None
+    if P.rotation_symmetric(g, prop):
+        try:
+            ss = D.GpuBicgStab(op, max_iter=20000, mixed=True)
+            pol_d = D.DeviceVector(ctx, op.vec_size, 64)
+            D.solve_particle(ss, params, prop, args.eps, pol_dev=pol_d)  # warm-up
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                r_ = D.solve_particle(ss, params, prop, args.eps, pol_dev=pol_d)
+            ctx.synchronize()
+            el_s = time.perf_counter() - t0
+            base = modes_out.get("fp64:qmr", {}).get("solves_per_s")
+            symmetric = {"workload": "the same sphere and direction, one mixed BiCGStab solve per particle "
+                                     "(polarisation 1; polarisation 2 by the quarter-turn symmetry)",
+                         "particles_per_s": args.steps / el_s,
+                         "vs_fp64_qmr_both_polarisations": (args.steps / el_s) / (base / 2) if base else None,
+                         "vs_fp64_qmr_one_polarisation": (args.steps / el_s) / base if base else None,
+                         "iterations": int(r_["infos"][0]["iterations"]),
+                         "cext": [float(c) for c in r_["cext"]], "cabs": [float(c) for c in r_["cabs"]]}
+            ss.close()
+            pol_d.close()
+        except Exception as e:
+            symmetric = {"error": str(e)}
 
     orient = None
     if args.orient_dirs > 0:
@@ -703,6 +745,7 @@ def main():
         "tuning": tuning,
         "device": info,
         "cpu_baseline": cpu,
+        "symmetric_single_solve": symmetric,
         "orientation_batch": orient,
         "c3_batch": c3,
         "serving": serving,

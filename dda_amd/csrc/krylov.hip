@@ -899,8 +899,10 @@ struct Core {
     gsig.clear();
   }
   std::vector<const void*> graph_signature() const {
-    std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat,
-                                  reinterpret_cast<const void*>((intptr_t)nrhs)};
+    // every buffer a captured launch references that can be reallocated
+    // (a larger block of right-hand sides, a pack's member table)
+    std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat, op->dims.member,
+                                  op->dims.dhat_tab, reinterpret_cast<const void*>((intptr_t)nrhs)};
     for (int t : {op->tune.col_wide[0], op->tune.col_wide[1], op->tune.p3_wide, op->tune.x_wide[0],
                   op->tune.x_wide[1], op->tune.split, (int)op->split_rhs})
       g.push_back(reinterpret_cast<const void*>((intptr_t)t));

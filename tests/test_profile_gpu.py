@@ -52,9 +52,12 @@ def test_profile_skips_launches_after_convergence(ctx):
     # a launch for part of the right-hand sides may land on either side of the
     # full-launch threshold in different passes
     assert counts.max() - counts.min() <= 2 and counts.min() > 0
-    # two matvecs per iteration: full launches while both right-hand sides
-    # run, none counted for the no-op ones queued past convergence
+    # two matvecs per iteration plus the start and the final true-residual
+    # matvec: full launches while both right-hand sides run, none counted for
+    # the no-op ones queued past convergence. A pass's events also enclose any
+    # wait of the GPU for the host's next launch (at a chunk boundary), which can
+    # lift a rare no-op launch over the threshold: two launches of slack.
     its = [i["iterations"] for i in infos]
-    assert 2 * min(its) <= counts[0] <= 2 * max(its) + 2
+    assert 2 * min(its) <= counts[0] <= 2 * max(its) + 4
     s.close()
     op.close()
