@@ -93,3 +93,35 @@ def test_symmetric_jones_on_chiral_target(ctx):
     orc = cases.oracle_problem(g, params, ccs[0])
     Jr = orc.jones(p[0], None, 1, *args)
     assert np.max(np.abs(J1 - Jr)) <= 1e-12 * np.max(np.abs(Jr))
+
+
+def test_anisotropic_material_solve_and_hdf5_output(ctx, tmp_path):
+    """An anisotropic (DiagMatrix3) material end to end: polarisabilities from
+    dda_couple_constants_diag (each axis its own index,
+    PolarizabilityDescription.cpp:69-75, 90-96), a device QMR-CS solve against
+    the oracle's, and the solution written as the reference's DipPol HDF5 file
+    and read back bit for bit."""
+    from dda_amd import hdf5 as H
+    g, params, _, _, prop, pol = cases.build("sphere_r4")
+    g.materials = [np.array([1.5 + 0.1j, 1.6 + 0.05j, 1.4 + 0.2j])]
+    cc, cs = D.couple_constants(g.materials, g.grid_unit, params.kd, prop, pol, "ldr")
+    assert not np.allclose(cs[0, 0], cs[0, 1])
+    op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, grid=params.grid)
+    op.set_couple_constants(cs)
+    einc = P.einc_plane(params, prop, pol)
+    s = D.GpuQmrCs(op, max_iter=2000)
+    p = s.get_pol_vec(einc, 1e-10)
+    orc = cases.oracle_problem(g, params, cs)
+    st, p_ref, _, _ = orc.solve("qmr", einc, 1e-10, max_iter=2000)
+    assert st == 0 and cases.rel_err(p, p_ref) <= 1e-8
+    assert abs(P.abs_cross(params, p, cc) / orc.abs_cross(p_ref, cc) - 1) <= 1e-8
+    assert not P.rotation_symmetric(g, prop)  # anisotropic: both polarisations are solved
+    if H.available():
+        f = tmp_path / "DipPol-Pol1.hdf5"
+        _, p1, p2 = P.beam_pols(prop)
+        H.write_dippol(f, params, p, 1, prop, [p1, p2])
+        assert np.array_equal(H.read_dippol(f, params), p)
+        mats = H.read_dataset(f, "/Geometry/RefractiveIndices", complex_=True)
+        np.testing.assert_array_equal(mats[0], g.materials[0])
+    s.close()
+    op.close()
