@@ -303,25 +303,27 @@ def serving_bench(args, D):
     out = {"workload": "seeded streams of %d jobs per load (burst of %d for capacity), sphere/box/ellipsoid, extent "
                        "U{16..96}, Re m U(1.2,2), Im m U(0,0.2), |m|kd=0.5, random orientation, 2 polarisations, "
                        "mixed BiCGStab to %g, 4 slots" % (args.serve_jobs, args.serve_burst, args.eps)}
+    # every stream is generated before any runs (no host threads competing
+    # with the submitting one)
     with ThreadPoolExecutor(4) as ex:
         f_burst = ex.submit(S.make_stream, args.serve_burst, 42)
         f_streams = [ex.submit(S.make_stream, args.serve_jobs, 43 + i) for i in range(len(loads))]
         burst_jobs = f_burst.result()
-        srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
-        try:
-            S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
-            burst = S.run_stream(srv, burst_jobs)
-            cap = burst["jobs"] / burst["makespan_s"]
-            out["capacity"] = dict(burst, capacity_jobs_per_s=cap)
-            for load, f in zip(loads, f_streams):
-                jobs = f.result()
-                r = S.run_stream(srv, jobs, rate=load * cap)
-                out["poisson_%.1fx" % load] = dict(r, rate_jobs_per_s=load * cap)
-                print("serving %.1fx: %s" % (load, json.dumps(r)), file=sys.stderr, flush=True)
-                del jobs
-            out["server"] = srv.stats()
-        finally:
-            srv.close()
+        streams = [f.result() for f in f_streams]
+    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
+    try:
+        S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
+        burst = S.run_stream(srv, burst_jobs)
+        cap = burst["jobs"] / burst["makespan_s"]
+        out["capacity"] = dict(burst, capacity_jobs_per_s=cap)
+        for i, load in enumerate(loads):
+            r = S.run_stream(srv, streams[i], rate=load * cap)
+            out["poisson_%.1fx" % load] = dict(r, rate_jobs_per_s=load * cap)
+            print("serving %.1fx: %s" % (load, json.dumps(r)), file=sys.stderr, flush=True)
+            streams[i] = None
+        out["server"] = srv.stats()
+    finally:
+        srv.close()
     # the burst again with different particles of one grid packed into one solve
     srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
     try:

@@ -105,7 +105,7 @@ def test_anisotropic_material_solve_and_hdf5_output(ctx, tmp_path):
     g, params, _, _, prop, pol = cases.build("sphere_r4")
     g.materials = [np.array([1.5 + 0.1j, 1.6 + 0.05j, 1.4 + 0.2j])]
     cc, cs = D.couple_constants(g.materials, g.grid_unit, params.kd, prop, pol, "ldr")
-    assert not np.allclose(cs[0, 0], cs[0, 1])
+    assert abs(cs[0, 0] - cs[0, 1]) > 1e-2 * abs(cs[0, 0])  # the axes differ
     op = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, grid=params.grid)
     op.set_couple_constants(cs)
     einc = P.einc_plane(params, prop, pol)
