@@ -40,7 +40,8 @@ EXPORTS = [
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
-    "dda_solver_set_inner_floor", "dda_solver_profile_read", "dda_solver_profiling_run",
+    "dda_solver_set_inner_floor", "dda_solver_set_inner_plateau", "dda_solver_profile_read",
+    "dda_solver_profiling_run",
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
     "dda_solver_solve_batch_dev", "dda_couple_constants", "dda_couple_constants_diag", "dda_incident_plane",
@@ -147,6 +148,7 @@ def lib():
         L.dda_solver_create.argtypes = [vp, i32, u64, u64, pp]
         L.dda_solver_create_ex.argtypes = [vp, i32, u64, u64, i32, pp]
         L.dda_solver_set_inner_floor.argtypes = [vp, dbl]
+        L.dda_solver_set_inner_plateau.argtypes = [vp, u64, dbl]
         L.dda_solver_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_destroy.argtypes = [vp]
         L.dda_solver_profiling_run.argtypes = [vp, dp, dp, dp]
@@ -543,11 +545,13 @@ class IterativeSolver:
     the operator may be precision 32 or 64 (an FP32 twin is built); "mixed"
     (or mixed=True) runs the Krylov iteration in FP32 inside FP64 iterative
     refinement (no counterpart in the reference); `inner_floor` is the
-    tolerance floor of the FP32 inner solves."""
+    tolerance floor of the FP32 inner solves, `inner_plateau` = (window,
+    ratio) their plateau stop."""
 
     kind = QMR_CS
 
-    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0, mixed=False, inner_floor=None, mode=None):
+    def __init__(self, gpu_matvec, max_iter=0, max_res_increase=0, mixed=False, inner_floor=None, mode=None,
+                 inner_plateau=None):
         self.op = gpu_matvec
         mode = mode if mode is not None else ("mixed" if mixed else "fp64")
         self.mode = mode
@@ -557,6 +561,8 @@ class IterativeSolver:
                                           MODES[mode], ctypes.byref(h)), "dda_solver_create")
         if inner_floor is not None:
             _check(lib().dda_solver_set_inner_floor(h, float(inner_floor)))
+        if inner_plateau is not None:  # (window, ratio)
+            _check(lib().dda_solver_set_inner_plateau(h, int(inner_plateau[0]), float(inner_plateau[1])))
         self.handle = h
         self.info = None
         gpu_matvec.ctx._adopt(self)

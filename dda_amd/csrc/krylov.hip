@@ -56,6 +56,11 @@ struct SolverState {
   // BiCGStab / CGNR / BiCG-CS
   double2 ro_new, ro_old, omega, bbeta, mbo, mOmega, one, mOne;
   double denumOmega, abs_ro_new, cg_ro_new, cg_ro_old;
+  // plateau stop of the mixed mode's inner solves (0 = off): STAGNATION once
+  // the smallest |r|^2 seen has not fallen below plateau_ratio times its value
+  // plateau_window iterations earlier
+  unsigned long long plateau_window, ckpt_count;
+  double plateau_ratio, ckpt_r;
 };
 constexpr int kStateInts = sizeof(SolverState) / sizeof(int);
 static_assert(sizeof(SolverState) % sizeof(int) == 0, "state stride");
@@ -553,6 +558,19 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
       }
       if (hist && st->count < st->hist_cap) hist[st->count] = st->residScale * ip1;
       loop_head(st, kind);
+      if (!st->done && st->plateau_window) {
+        if (st->ckpt_count == 0 && st->ckpt_r == 0.0) {
+          st->ckpt_r = st->inprodR;
+          st->ckpt_count = st->count;
+        } else if (st->count - st->ckpt_count >= st->plateau_window) {
+          if (st->inprodR > st->plateau_ratio * st->ckpt_r) {
+            finish(st, DDA_STAGNATION);
+          } else {
+            st->ckpt_r = st->inprodR;
+            st->ckpt_count = st->count;
+          }
+        }
+      }
       break;
     }
     // ---- BiCGStab, BicgStab.cpp:66-112 ----
@@ -853,6 +871,9 @@ struct Core {
   // anything its launches reference changes.
   hipGraphExec_t gexec = nullptr;
   std::vector<const void*> gsig;
+  // plateau stop for the next solves (SolverState::plateau_window; 0 = off)
+  uint64_t plateau_window = 0;
+  double plateau_ratio = 0.81;
 
   dim3 g() const { return dim3(nb, nrhs); }
   dim3 b() const { return dim3(kRedThreads); }
@@ -946,6 +967,8 @@ struct Core {
       std::memset(&h[i], 0, sizeof(SolverState));
       h[i].max_iter = max_iters ? max_iters[i] : max_iter;
       h[i].max_res_increase = max_res_increase;
+      h[i].plateau_window = plateau_window;
+      h[i].plateau_ratio = plateau_ratio;
       h[i].hist_cap = hist_cap;
       h[i].eps = eps[i];
       if (active && !active[i]) {
@@ -1205,6 +1228,8 @@ struct dda_solver_s {
   uint64_t last_iters = 0;
   std::vector<double> outer_hist;
   double inner_floor = 1e-4;
+  uint64_t inner_plateau = 0;     // set at creation: kInnerPlateau
+  double inner_plateau_ratio = 0.81;
 };
 
 namespace {
@@ -1378,8 +1403,14 @@ dda_status run_fp32(dda_solver_s* s, int k, double eps, bool have_start, dda_sol
 //     sum_i E_i^2 over the particle does this);
 //   - after an FP32 step that did not halve the FP64 residual, that
 //     right-hand side stays on FP64 for the rest of the solve.
+// An FP32 inner solve also stops early (its status STAGNATION, its correction
+// applied) once the smallest residual it has seen fell by less than 10% over
+// kInnerPlateau iterations: the outer step then restarts it from the FP64
+// residual. (FP32 QMR-CS on the 512-across sphere stalled at |r| / |b| =
+// 1.3e-2 for 3000 iterations without it.)
 // Right-hand sides that have converged are masked out of the later steps.
 constexpr double kRhoMin = 1e-4;
+constexpr uint64_t kInnerPlateau = 200;
 
 dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_solve_info* info) {
   dda_op_s* op = s->op;
@@ -1465,6 +1496,8 @@ dda_status run_mixed(dda_solver_s* s, int k, double eps, bool have_start, dda_so
       if (pass == 0) {
         hipLaunchKernelGGL(k_load_rhs<float2>, G, B, 0, st, (float2*)s->c32.v[0], s->r64, s->n, s->scale, s->mask);
         DDA_TRY_HIP(hipGetLastError());
+        s->c32.plateau_window = s->inner_plateau;
+        s->c32.plateau_ratio = s->inner_plateau_ratio;
         DDA_TRY(s->c32.solve(k, eps_in.data(), act.data(), false, fin.data(), budget.data()));
       } else {
         if (s->c64.cap < s->cap) DDA_TRY(s->c64.alloc(op, s->kind, s->max_iter, s->max_res_increase, s->cap));
@@ -1540,6 +1573,7 @@ extern "C" dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_i
   s->max_iter = max_iter ? max_iter : 3ull * d.Bx * d.By * d.Bz;  // DDA/DDAMain.cpp:640-644
   static const uint64_t defaults[4] = {50000, 30000, 10, 50000};  // QmrCs.cpp:39, BicgStab.cpp:51, Cgnr.cpp:34
   s->max_res_increase = max_res_increase ? max_res_increase : defaults[kind];
+  s->inner_plateau = kInnerPlateau;
   s->n = 3ll * d.S;
   dda_status rc = DDA_OK;
   if (mode == DDA_MODE_MIXED || (mode == DDA_MODE_FP32 && op->prec == 64)) {
@@ -1676,6 +1710,16 @@ extern "C" dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_) 
     return DDA_INVALID_ARG;
   }
   s->inner_floor = floor_;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_solver_set_inner_plateau(dda_solver_t s, uint64_t window, double ratio) {
+  if (!s || s->mode != DDA_MODE_MIXED || !(ratio > 0 && ratio <= 1)) {
+    set_error("dda_solver_set_inner_plateau: a mixed-mode solver and a ratio in (0, 1]");
+    return DDA_INVALID_ARG;
+  }
+  s->inner_plateau = window;
+  s->inner_plateau_ratio = ratio;
   return DDA_OK;
 }
 

@@ -258,6 +258,12 @@ dda_status dda_solver_create_ex(dda_op_t op, int kind, uint64_t max_iter, uint64
  * residual in dda_solve_info is then recomputed with that FP32 operator. */
 /* Tolerance floor of the FP32 inner solves in mixed mode (default 1e-4). */
 dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
+/* Plateau stop of the FP32 inner solves in mixed mode: an inner solve ends
+ * (its correction applied, the outer step restarting it from the FP64
+ * residual) once the smallest |r|^2 it has seen did not fall below ratio times
+ * its value `window` iterations earlier. Default window 200, ratio 0.81 (|r|
+ * down by less than 10%); window 0 turns it off. */
+dda_status dda_solver_set_inner_plateau(dda_solver_t s, uint64_t window, double ratio);
 /* dda_op_profile_read for the operator the Krylov iteration runs on: the FP32
  * twin in mixed mode (it follows the FP64 operator's dda_op_profile switch),
  * else the FP64 operator itself. */

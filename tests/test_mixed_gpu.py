@@ -124,3 +124,27 @@ def test_mixed_near_breakdown_rhs(ctx, kind):
     assert s.info["true_rel_residual"] < 1e-6 and s.info["iterations"] < 1000
     c, c64 = P.ext_cross(params, p, b), P.ext_cross(params, p64, b)
     assert abs(c - c64) <= 1e-5 * abs(c64)
+
+
+def test_inner_plateau_stop_restarts_from_fp64_residual(ctx):
+    """The mixed mode's plateau stop, forced from the host with a window of 8
+    iterations and a ratio no solve can meet: every FP32 inner solve ends after
+    ~8 iterations with its correction applied, and the outer FP64 steps
+    restart it until the solve meets eps; the answer matches the FP64 solve.
+    Invalid settings are refused."""
+    g, params, cc, cs, prop, pol = cases.build("sphere_r7")
+    einc = P.einc_plane(params, prop, pol)
+    op = _op(ctx, g, params, cs)
+    p64 = D.GpuQmrCs(op, max_iter=3000).get_pol_vec(einc, 1e-10)
+    free = D.GpuQmrCs(op, max_iter=3000, mixed=True)
+    p_free = free.get_pol_vec(einc, 1e-8)
+    forced = D.GpuQmrCs(op, max_iter=3000, mixed=True, inner_plateau=(8, 1e-30))
+    p = forced.get_pol_vec(einc, 1e-8)
+    print("free", free.info, "forced", forced.info, flush=True)
+    assert forced.info["true_rel_residual"] < 1e-8
+    assert forced.info["refinements"] > free.info["refinements"]
+    assert cases.rel_err(p, p64) <= 1e-6 and cases.rel_err(p_free, p64) <= 1e-6
+    with pytest.raises(D.DDAError):
+        D.GpuQmrCs(op, inner_plateau=(8, 0.5))  # not a mixed-mode solver
+    with pytest.raises(D.DDAError):
+        D.GpuQmrCs(op, mixed=True, inner_plateau=(8, 0.0))
