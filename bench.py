@@ -293,16 +293,18 @@ def _plain(o, path="line"):
 def serving_bench(args, D):
     """The serving stream: jobs of random shape (sphere / box / ellipsoid, largest
     extent 16..96 dipoles), Re m in 1.2..2, Im m in 0..0.2, |m| kd = 0.5, random
-    orientation, both polarisations, mixed BiCGStab to eps, 4 slots. Capacity
-    from a burst of jobs submitted at once; then fresh seeded streams with
-    Poisson arrivals at 0.5x, 0.9x and 1.2x that capacity; latency = completion
-    - submission. The streams are generated on host threads ahead of use."""
+    orientation, both polarisations, mixed BiCGStab to eps, args.serve_slots
+    slots. Capacity from a burst of jobs submitted at once; then fresh seeded
+    streams with Poisson arrivals at 0.5x, 0.9x and 1.2x that capacity; latency
+    = completion - submission. The streams are generated on host threads ahead
+    of use. Runs in its own process (--serve-only), as a server would."""
     from concurrent.futures import ThreadPoolExecutor
     from dda_amd import serving as S
     loads = (0.5, 0.9, 1.2)
     out = {"workload": "seeded streams of %d jobs per load (burst of %d for capacity), sphere/box/ellipsoid, extent "
                        "U{16..96}, Re m U(1.2,2), Im m U(0,0.2), |m|kd=0.5, random orientation, 2 polarisations, "
-                       "mixed BiCGStab to %g, 4 slots" % (args.serve_jobs, args.serve_burst, args.eps)}
+                       "mixed BiCGStab to %g, %d slots" % (args.serve_jobs, args.serve_burst, args.eps,
+                                                            args.serve_slots)}
     # every stream is generated before any runs (no host threads competing
     # with the submitting one)
     with ThreadPoolExecutor(4) as ex:
@@ -310,7 +312,7 @@ def serving_bench(args, D):
         f_streams = [ex.submit(S.make_stream, args.serve_jobs, 43 + i) for i in range(len(loads))]
         burst_jobs = f_burst.result()
         streams = [f.result() for f in f_streams]
-    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps)
+    srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
         burst = S.run_stream(srv, burst_jobs)
@@ -325,7 +327,7 @@ def serving_bench(args, D):
     finally:
         srv.close()
     # the burst again with different particles of one grid packed into one solve
-    srv = D.Server(slots=4, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
+    srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))
         out["capacity_pack_grids"] = S.run_stream(srv, burst_jobs)
@@ -334,6 +336,20 @@ def serving_bench(args, D):
     ref = out["poisson_0.9x"]
     out.update({"p50_ms": ref["p50_ms"], "p99_ms": ref["p99_ms"], "solves_per_s": ref["solves_per_s"]})
     return out
+
+
+def serving_child(args):
+    """serving_bench in a child process of its own (a fresh HIP runtime with no
+    other contexts or streams, as a serving process has)."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", "--eps", repr(args.eps),
+           "--serve-jobs", str(args.serve_jobs), "--serve-burst", str(args.serve_burst),
+           "--serve-slots", str(args.serve_slots)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "serving child exited with %d" % r.returncode}
+    return json.loads(lines[-1])
 
 
 def main():
@@ -363,6 +379,9 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=200,
                     help="jobs of each seeded Poisson serving stream (0 = skip the serving measurement)")
     ap.add_argument("--serve-burst", type=int, default=48, help="jobs of the capacity burst")
+    ap.add_argument("--serve-slots", type=int, default=4, help="worker slots of the serving measurement")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving measurement (one JSON line); the full run starts itself this way")
     ap.add_argument("--no-large-grid", action="store_true",
                     help="skip the 1024^3 (sphere 512 across) FP32 / FP64 matvec figures")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
@@ -377,6 +396,9 @@ def main():
     import numpy as np
     import dda_amd as D
     from dda_amd import problem as P
+    if args.serve_only:
+        print(json.dumps(_plain(serving_bench(args, D))))
+        return 0
     if args.large:
         return large_bench(args, D, P, np)
 
@@ -675,7 +697,7 @@ def main():
     serving = None
     if args.serve_jobs > 0:
         try:
-            serving = serving_bench(args, D)
+            serving = serving_child(args)
         except Exception as e:  # reported, not fatal: the headline is the solve rate of the 128-across sphere
             serving = {"error": str(e)}
 

@@ -41,6 +41,17 @@ template <typename C> __host__ __device__ inline C cmul_mi(C a) { C r; r.x = a.y
 template <typename C> __host__ __device__ inline C cmul_pi(C a) { C r; r.x = -a.y; r.y = a.x; return r; }
 template <typename C> __host__ __device__ inline C czero() { C r; r.x = 0; r.y = 0; return r; }
 
+__host__ __device__ inline float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ inline double ffma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// a0 x0 + a1 x1 + a2 x2 as one chain of fused multiply-adds per part (12
+// instructions; products and sums written separately take 16)
+template <typename C> __host__ __device__ inline C cdot3(C a0, C x0, C a1, C x1, C a2, C x2) {
+  C r;
+  r.x = ffma(a0.x, x0.x, ffma(-a0.y, x0.y, ffma(a1.x, x1.x, ffma(-a1.y, x1.y, ffma(a2.x, x2.x, -a2.y * x2.y)))));
+  r.y = ffma(a0.x, x0.y, ffma(a0.y, x0.x, ffma(a1.x, x1.y, ffma(a1.y, x1.x, ffma(a2.x, x2.y, a2.y * x2.x)))));
+  return r;
+}
+
 // Division of n < 2^31 by an invariant divisor d >= 1 with one multiply-high:
 // q = (umulhi(n, m) + n) >> s, m = floor(2^32 (2^s - d) / d) + 1, s = ceil(log2 d)
 // (Granlund & Montgomery, "Division by invariant integers using multiplication", 1994).

@@ -297,9 +297,9 @@ __global__ __launch_bounds__(256) void k_pass3(typename CxT<R>::T* a2, const typ
         const C d0 = D[0], d3 = D[3 * oct], d5 = D[5 * oct];
         const C d1 = cscale(D[oct], sx * sy), d2 = cscale(D[2 * oct], sx * sz), d4 = cscale(D[4 * oct], sy * sz);
         const C x0 = reg[0][k], x1 = reg[1][k], x2 = reg[2][k];
-        reg[0][k] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
-        reg[1][k] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
-        reg[2][k] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
+        reg[0][k] = cdot3(d0, x0, d1, x1, d2, x2);
+        reg[1][k] = cdot3(d1, x0, d3, x1, d4, x2);
+        reg[2][k] = cdot3(d2, x0, d4, x1, d5, x2);
       }
     }
   }
@@ -469,6 +469,8 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, N, PL::PPT>::value))) void k_pass3
   const R sx = kx <= hx ? (R)1 : (R)-1;
   const uint32_t fyx = colok ? (uint32_t)fy * (hx + 1) + fx : 0u;
   const R sxy = sx * sy;
+  // (issuing point e + 1's tensor loads before point e is formed was measured
+  // slower: 275 -> 310 us per 2-RHS FP32 launch at 256^3, 300 -> 347 us FP64)
 #pragma unroll
   for (int e = 0; e < PPT; e++) {
     const int kz = first_point<N, PPT, Inv>(t, e);
@@ -481,9 +483,9 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, N, PL::PPT>::value))) void k_pass3
     const C d0 = dc[0], d1 = cscale(dc[1], sxy), d2 = cscale(dc[2], sx * sz), d3 = dc[3], d4 = cscale(dc[4], sy * sz),
             d5 = dc[5];
     const C x0 = reg[0][e], x1 = reg[1][e], x2 = reg[2][e];
-    reg[0][e] = cadd(cadd(cmul(d0, x0), cmul(d1, x1)), cmul(d2, x2));
-    reg[1][e] = cadd(cadd(cmul(d1, x0), cmul(d3, x1)), cmul(d4, x2));
-    reg[2][e] = cadd(cadd(cmul(d2, x0), cmul(d4, x1)), cmul(d5, x2));
+    reg[0][e] = cdot3(d0, x0, d1, x1, d2, x2);
+    reg[1][e] = cdot3(d1, x0, d3, x1, d4, x2);
+    reg[2][e] = cdot3(d2, x0, d4, x1, d5, x2);
   }
   // the store resources are formed anew (an opaque copy of the base keeps the
   // compiler from holding the load ones in scalar registers across the
@@ -982,6 +984,14 @@ dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, 
 // FP32 only: in FP64 the 32-point plan's data alone needs 384 VGPRs. The FP32
 // default at 1024 points is 2: measured at 1024^3 (sphere 512 across) 14.8 ms
 // against 17.8 ms (wide), 19.2 ms (variant 3) and 33.0 ms (narrow).
+// (Two further forms were measured and dropped. The wide plan without held
+// twiddle powers at 4 waves per SIMD, 128 VGPRs: 308.8 against 293.2 us per
+// 2-RHS FP32 launch at 256^3, and it spills at 32, 128 and 512 points. A
+// persistent kernel, resident work-groups looping over the tiles and loading
+// each component of the next tile into the registers its store frees:
+// 303.5 against 289.3 us (2-RHS FP32, 256^3), 315.0 against 291.4 us (FP64),
+// 15.1 against 14.6 ms (FP32, 1024^3); its loop costs 13% more VALU at 1024
+// points than the hidden loads save.)
 typedef PlanOf<1024, 32, RadixList<32, 32>> Plan1024x32;
 inline int p3_variants(int n, int prec) { return n == 1024 && prec == 32 ? 4 : 2; }
 
@@ -2399,8 +2409,11 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
     const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : 2;
+    // two rounds over the variants, the faster of each variant's two timings
     double t[4] = {0, 0, 0, 0};
-    for (int w = 0; w < nvar; w++) {
+    for (int it = 0; it < 2 * nvar; it++) {
+      const int w = it % nvar;
+      double tt = 0;
       dda_status rc;
       if (pass >= 3) {
         op->tune.x_wide[pass - 3] = w;
@@ -2416,7 +2429,7 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
             default:
               return DDA_NOT_SUPPORTED;
           }
-        }, &t[w]);
+        }, &tt);
       } else if (pass == 1) {
         op->tune.p3_wide = w;
         rc = time_pass(st, reps, [&]() {
@@ -2429,7 +2442,7 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
             default:
               return DDA_NOT_SUPPORTED;
           }
-        }, &t[w]);
+        }, &tt);
       } else {
         const bool inv = pass == 2;
         ColParams cp;
@@ -2446,17 +2459,22 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
         rc = time_pass(st, reps, [&]() {
           return inv ? launch_col_any<R, true>(in, out, cp, op->plan[1], op->col_t, op->tw[1], nb, st, RhsFlags(), w)
                      : launch_col_any<R, false>(in, out, cp, op->plan[1], op->col_t, op->tw[1], nb, st, RhsFlags(), w);
-        }, &t[w]);
+        }, &tt);
       }
       if (rc != DDA_OK) {
         op->tune = best;
         if (xv) (void)hipFree(xv);
         return rc;
       }
+      t[w] = it < nvar ? tt : std::min(t[w], tt);
     }
-    int pick = t[1] <= t[0] ? 1 : 0;
-    for (int w = 2; w < nvar; w++)
-      if (t[w] < t[pick]) pick = w;
+    // the precision's default unless another variant is faster by more than
+    // 2% (closer timings change places between runs)
+    const int def = pass == 1 && d.Gz == 1024 && sizeof(R) == 4 ? 2 : sizeof(R) == 4 ? 1 : 0;
+    int fastest = 0;
+    for (int w = 1; w < nvar; w++)
+      if (t[w] < t[fastest]) fastest = w;
+    const int pick = t[fastest] < 0.98 * t[def] ? fastest : def;
     if (pass == 1)
       best.p3_wide = pick;
     else if (pass >= 3)

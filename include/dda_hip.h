@@ -178,9 +178,11 @@ dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs
 dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
 /* Autotuner (no counterpart in the reference, whose FFT plans are fixed per
  * length): times each variant of the column passes (plan width: 8 or 16
- * points per thread; for 1024-point Z columns also the 32 x 32 plan over 16 or
- * 8 columns per work-group) on this operator's own buffers for nrhs
- * right-hand sides and keeps the fastest per pass. The choice is cached for the
+ * points per thread; for 1024-point FP32 Z columns also the 32 x 32 plan
+ * over 16 or 8 columns per work-group) on this operator's own buffers for nrhs
+ * right-hand sides, each twice (the faster timing counts), and keeps per pass
+ * the fastest where it beats the precision's default by more than 2% (the
+ * default otherwise). The choice is cached for the
  * process by (grid, box, precision), so later operators of that shape start
  * tuned, and the FP32 twin of a mixed-mode solver over a tuned operator is tuned
  * too. Results differ between variants only by rounding. `report` (nullable)
