@@ -116,6 +116,47 @@ def matvec_profile(ctx, op64, g, params, rhs, k, D, P, np, nrhs=1):
     return 0
 
 
+def matvec_leg(ctx, op64, g, params, rhs, D, P, np):
+    """Per-apply times of the workload's matvec: FP64 and FP32 with one
+    right-hand side, FP32 with both polarisations as one block. x: nv entries
+    per component with re, im ~ U[-1, 1) (numpy default_rng, seed 0x5EED)."""
+    out = {"method": "3 warm-up applies, then 20 applies, each timed as the sum of its five kernels' "
+                     "HIP-event durations on the operator's stream"}
+    op32 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=32, grid=params.grid)
+    try:
+        op32.set_couple_constants_batch(np.array([r[2] for r in rhs]))
+        op32.tune(2)
+        x1 = P.random_dipole_vector(params)
+        x2 = np.concatenate([x1, P.random_dipole_vector(params, seed=0x5EED + 1)])
+        for name, op, prec, nrhs in (("fp64_1rhs", op64, 64, 1), ("fp32_1rhs", op32, 32, 1),
+                                     ("fp32_2rhs", op32, 32, 2)):
+            x = D.DeviceVector(ctx, nrhs * op.vec_size, prec)
+            y = D.DeviceVector(ctx, nrhs * op.vec_size, prec)
+            x.upload((x2 if nrhs == 2 else x1).astype(np.complex128 if prec == 64 else np.complex64))
+            if op is op64:
+                op.set_couple_constants(rhs[0][2])
+            apply = (lambda: op.apply_batch(x, y, nrhs)) if nrhs > 1 else (lambda: op.apply(x, y))
+            for _ in range(3):
+                apply()
+            ctx.synchronize()
+            op.profile(True)
+            op.profile_read()
+            us = []
+            for _ in range(20):
+                apply()
+                ctx.synchronize()
+                ms, cnt, _ = op.profile_read()
+                us.append(1e3 * float(np.sum(ms)))
+            op.profile(False)
+            x.close()
+            y.close()
+            out[name] = {"median_us": float(np.median(us)), "p10_us": float(np.percentile(us, 10)),
+                         "p90_us": float(np.percentile(us, 90))}
+    finally:
+        op32.close()
+    return out
+
+
 def c3_bench(args, D, P, np, ctx):
     """Batched right-hand sides: ellipsoid with semi-axes 32x24x16 dipoles,
     m = 1.33+0.01i, |m| kd = 0.6, 64 seeded directions x 2 polarisations = 128
@@ -657,6 +698,14 @@ def main():
         except Exception as e:
             symmetric = {"error": str(e)}
 
+    # per-apply matvec times: 3 warm-ups, then 20 applies, each the sum of its
+    # five kernels' HIP-event durations (median, p10, p90)
+    matvec_applies = None
+    try:
+        matvec_applies = matvec_leg(ctx, op, g, params, rhs, D, P, np)
+    except Exception as e:  # reported, not fatal
+        matvec_applies = {"error": str(e)}
+
     orient = None
     if args.orient_dirs > 0:
         try:
@@ -770,6 +819,7 @@ def main():
         "device": info,
         "cpu_baseline": cpu,
         "symmetric_single_solve": symmetric,
+        "matvec_applies": matvec_applies,
         "orientation_batch": orient,
         "c3_batch": c3,
         "serving": serving,

@@ -598,21 +598,34 @@ __global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __rest
   C v[PPT];
 #pragma unroll
   for (int e = 0; e < PPT; e++) v[e] = ok ? src[first_point<N, PPT, F>(t, e)] : czero<C>();
+  // the dipoles this thread gathers (a row holds at most N/2: the grid spans
+  // at least twice the box), their x entries and positions loaded before the
+  // transform so that those loads wait while it runs
+  constexpr int MAXD = (N / 2 + TPC - 1) / TPC;
+  const uint32_t s0 = ok ? row_start[row] : 0u, s1 = ok ? row_start[row + 1] : 0u;
+  uint32_t gi[MAXD], gm[MAXD];
+  C ga[MAXD];
+#pragma unroll
+  for (int k = 0; k < MAXD; k++) {
+    const uint32_t q = s0 + t + k * TPC;
+    gi[k] = q < s1 ? (d.sorted ? q : row_dip[q]) : 0u;
+    gm[k] = q < s1 ? row_meta[q] : 0u;
+    ga[k] = q < s1 ? x[(size_t)c * d.S + gi[k]] : czero<C>();
+  }
   RunStages<N, PPT, true, 1, C, F>::run(v, t, tile, lp, l, tw);
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < PPT; e++)
     if (!upper_half<Out>(e)) tile[first_point<N, PPT, Out>(t, e) * lp + l] = v[e];
   __syncthreads();
-  if (ok) {
-    const uint32_t s0 = row_start[row], s1 = row_start[row + 1];
-    for (uint32_t q = s0 + t; q < s1; q += TPC) {
-      const uint32_t i = d.sorted ? q : row_dip[q], meta = row_meta[q];
-      C a = x[(size_t)c * d.S + i];
+#pragma unroll
+  for (int k = 0; k < MAXD; k++) {
+    if (s0 + t + k * TPC < s1) {
+      C a = ga[k];
       if (conj) a = cconj(a);
-      C r = cadd(cmul(ccs[3 * (meta >> 16) + c], tile[(meta & 0xffff) * lp + l]), a);
+      C r = cadd(cmul(ccs[3 * (gm[k] >> 16) + c], tile[(gm[k] & 0xffff) * lp + l]), a);
       if (conj) r = cconj(r);
-      y[(size_t)c * d.S + i] = r;
+      y[(size_t)c * d.S + gi[k]] = r;
     }
   }
   if (blockIdx.x == 0) {
