@@ -406,6 +406,8 @@ def main():
     ap.add_argument("--diameter", type=int, default=128, help="sphere diameter in dipoles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 128-RHS ellipsoid batch figure")
+    ap.add_argument("--no-matvec-applies", action="store_true",
+                    help="skip the per-apply matvec timing (median, p10, p90)")
     ap.add_argument("--host-step", action="store_true",
                     help="also time one step through the host-array path (PCIe copies included)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default kernel variants)")
@@ -701,10 +703,11 @@ def main():
     # per-apply matvec times: 3 warm-ups, then 20 applies, each the sum of its
     # five kernels' HIP-event durations (median, p10, p90)
     matvec_applies = None
-    try:
-        matvec_applies = matvec_leg(ctx, op, g, params, rhs, D, P, np)
-    except Exception as e:  # reported, not fatal
-        matvec_applies = {"error": str(e)}
+    if not args.no_matvec_applies:
+        try:
+            matvec_applies = matvec_leg(ctx, op, g, params, rhs, D, P, np)
+        except Exception as e:  # reported, not fatal
+            matvec_applies = {"error": str(e)}
 
     orient = None
     if args.orient_dirs > 0:
