@@ -83,6 +83,8 @@ dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj
 // entry point (observe.hip) for `device`, which must be current; once per
 // device, later calls return at once. dda_ctx_create calls it.
 constexpr int kMaxNoAllocDevices = 64;
+// Blocks of at least this many right-hand sides have a tuning of their own.
+constexpr int kBlockTuneRhs = 8;
 dda_status noalloc_scratch_setup(int device);
 
 }  // namespace dda
@@ -156,6 +158,10 @@ struct dda_op_s {
     int split = -1;              // a block as two half-blocks on two streams: -1 default (off), 0 off, 1 on
   } tune;
   bool tuned = false;
+  // the tuning for blocks of kBlockTuneRhs or more right-hand sides (used when
+  // tuned_blk; else `tune` serves every block size)
+  Tune tune_blk;
+  bool tuned_blk = false;
   struct dda_dhat_cache_s* dhat_cache = nullptr;  // set when dhat is a cache entry (not owned)
   // A pack of several particles on one grid (dda_op_create_pack): the union
   // of their positions, each member's materials on it (void = nmat - 1,

@@ -924,9 +924,10 @@ struct Core {
     // (a larger block of right-hand sides, a pack's member table)
     std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat, op->dims.member,
                                   op->dims.dhat_tab, reinterpret_cast<const void*>((intptr_t)nrhs)};
-    for (int t : {op->tune.col_wide[0], op->tune.col_wide[1], op->tune.p3_wide, op->tune.x_wide[0],
-                  op->tune.x_wide[1], op->tune.split, (int)op->split_rhs})
-      g.push_back(reinterpret_cast<const void*>((intptr_t)t));
+    for (const dda_op_s::Tune* tn : {&op->tune, &op->tune_blk})
+      for (int t : {tn->col_wide[0], tn->col_wide[1], tn->p3_wide, tn->x_wide[0], tn->x_wide[1], tn->split})
+        g.push_back(reinterpret_cast<const void*>((intptr_t)t));
+    for (int t : {(int)op->split_rhs, (int)op->tuned_blk}) g.push_back(reinterpret_cast<const void*>((intptr_t)t));
     for (C* p : v) g.push_back(p);
     return g;
   }
@@ -1632,7 +1633,6 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
     for (int r = 0; r < nrhs; r++) rep.insert(rep.end(), op->cc_sqrt_host.begin(), op->cc_sqrt_host.end());
     DDA_TRY(dda_op_set_cc_sqrt_batch(op, rep.data(), nrhs));
   }
-  auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = op->ctx->stream;
   DDA_TRY(alloc_buffers(s, nrhs));
   if (s->op32) {  // the FP32 twin of the mixed and FP32 modes follows the caller's operator
@@ -1642,6 +1642,11 @@ static dda_status solve_batch(dda_solver_t s, const void* einc, int nrhs, double
     s->op32->tune.split = op->tune.split;  // and its block split choice
     s->op32->split_rhs = op->split_rhs;
   }
+  // a block of kBlockTuneRhs or more over a tuned operator: the operator that
+  // runs the iterations is tuned for blocks too, once per shape (cached)
+  dda_op_s* iter_op = s->op32 ? s->op32 : op;
+  if (op->tuned && nrhs >= kBlockTuneRhs && !iter_op->tuned_blk) DDA_TRY(dda_op_tune(iter_op, nrhs, nullptr, 0));
+  auto t0 = std::chrono::steady_clock::now();
   const size_t bytes = (size_t)nrhs * s->n * sizeof(double2);
   const hipMemcpyKind in_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   DDA_TRY_HIP(hipMemcpyAsync(s->io, einc, bytes, in_kind, st));

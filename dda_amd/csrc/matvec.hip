@@ -909,19 +909,35 @@ inline bool plan_wide(int w) {
   return w < 0 ? sizeof(R) == 4 : w != 0;
 }
 
-template <typename R, int N, bool W, bool INV>
+// The other tile the tuner tries for the FP32 Y passes: twice the default
+// columns per work-group where that stays within 1024 threads, else half
+// (keeping at least one wavefront).
+template <int T, int TPC>
+constexpr int other_cols() {
+  return 2 * T * TPC <= 1024 ? 2 * T : (T / 2) * TPC >= 64 ? T / 2 : T;
+}
+
+template <typename R, int N, bool W, bool INV, bool ALT = false>
 dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch,
                                hipStream_t st, RhsFlags done) {
   typedef typename CxT<R>::T C;
-  constexpr int T = static_cols<R, N, W, 1024>();
+  constexpr int T0 = static_cols<R, N, W, 1024>();
+  constexpr int T = ALT ? other_cols<T0, N / StaticPlan<N, W>::PPT>() : T0;
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const size_t lds = (size_t)N * lp * sizeof(C);
   // the matvec's forward pass reads at most N/2 points and its inverse pass
   // keeps at most N/2; the tensor generation transforms full lengths
   const bool hin = !INV && 2 * cp.in_len <= N, hout = INV && 2 * cp.out_len <= N;
-  auto k = hin ? k_colfft_s<R, N, T, W, INV, !INV, false>
-               : hout ? k_colfft_s<R, N, T, W, INV, false, INV> : k_colfft_s<R, N, T, W, INV, false, false>;
+  void (*k)(const C*, C*, ColParams, const C*, RhsFlags);
+  if constexpr (ALT) {
+    // the other tile is a matvec variant: full-length transforms keep the default
+    if (!hin && !hout) return launch_col_static_w<R, N, W, INV, false>(in, out, cp, tw, nbatch, st, done);
+    k = INV ? k_colfft_s<R, N, T, W, INV, false, INV> : k_colfft_s<R, N, T, W, INV, !INV, false>;
+  } else {
+    k = hin ? k_colfft_s<R, N, T, W, INV, !INV, false>
+            : hout ? k_colfft_s<R, N, T, W, INV, false, INV> : k_colfft_s<R, N, T, W, INV, false, false>;
+  }
   DDA_TRY(set_lds(k, lds));
   hipLaunchKernelGGL(k, dim3((cp.ncols + T - 1) / T, nbatch), dim3(threads), lds, st, (const C*)in, (C*)out, cp,
                      (const C*)tw, done);
@@ -929,11 +945,18 @@ dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, c
   return DDA_OK;
 }
 
+// wide: -1 the precision's default plan, 0 narrow, 1 wide; FP32 also 2 and 3,
+// narrow and wide with the other tile (other_cols).
 template <typename R, int N, bool INV>
 dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
                              RhsFlags done, int wide) {
-  return plan_wide<R>(wide) ? launch_col_static_w<R, N, true, INV>(in, out, cp, tw, nbatch, st, done)
-                            : launch_col_static_w<R, N, false, INV>(in, out, cp, tw, nbatch, st, done);
+  if constexpr (sizeof(R) == 4) {
+    if (wide >= 2)
+      return (wide & 1) ? launch_col_static_w<R, N, true, INV, true>(in, out, cp, tw, nbatch, st, done)
+                        : launch_col_static_w<R, N, false, INV, true>(in, out, cp, tw, nbatch, st, done);
+  }
+  return plan_wide<R>(wide >= 2 ? (wide & 1) : wide) ? launch_col_static_w<R, N, true, INV>(in, out, cp, tw, nbatch, st, done)
+                                                    : launch_col_static_w<R, N, false, INV>(in, out, cp, tw, nbatch, st, done);
 }
 
 // The buffers of right-hand sides r0 .. r0 + nrhs - 1 of an operator: what one
@@ -976,7 +999,15 @@ dda_status launch_pass3_plan(dda_op_s* op, const MvView& v, hipStream_t st, RhsF
   return DDA_OK;
 }
 
-template <typename R, int N, bool W>
+// The other tile the tuner tries for the FP32 Z pass: 32 columns where the
+// default is 16 and 32 fit the work-group size the kernel's register budget
+// allows (MAXTH threads), else half the default.
+template <int T, int TPC, int MAXTH>
+constexpr int other_cols_z() {
+  return T != 32 && 32 * TPC <= MAXTH ? 32 : (T / 2) * TPC >= 64 ? T / 2 : T;
+}
+
+template <typename R, int N, bool W, bool ALT = false>
 dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
   // FP32 1024-point columns: 16 columns (128-byte segments) in a 1024-thread
   // work-group at 128 VGPRs, the CU's whole register file (17.5 ms at 1024^3,
@@ -984,8 +1015,10 @@ dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, 
   // short FP32 columns (at most 64 points): 32 columns (256-byte segments)
   // per work-group; measured on a 128-RHS block at a 64-point Z grid: 935 ->
   // 781 us per launch
-  constexpr int T = static_cols<R, N, W, (N == 1024 && sizeof(R) == 4 && W) ? 1024 : 512,
-                                (sizeof(R) == 4 && N <= 64) ? 32 : 0>();
+  constexpr int T0 = static_cols<R, N, W, (N == 1024 && sizeof(R) == 4 && W) ? 1024 : 512,
+                                 (sizeof(R) == 4 && N <= 64) ? 32 : 0>();
+  constexpr int TPC = N / StaticPlan<N, W>::PPT;
+  constexpr int T = ALT ? other_cols_z<T0, TPC, 256 * P3Waves<R, N, StaticPlan<N, W>::PPT>::value>() : T0;
   return launch_pass3_plan<R, N, StaticPlan<N, W>, T>(op, v, st, done, nrhs);
 }
 
@@ -1006,16 +1039,23 @@ dda_status launch_pass3_static_w(dda_op_s* op, const MvView& v, hipStream_t st, 
 // 15.1 against 14.6 ms (FP32, 1024^3); its loop costs 13% more VALU at 1024
 // points than the hidden loads save.)
 typedef PlanOf<1024, 32, RadixList<32, 32>> Plan1024x32;
-inline int p3_variants(int n, int prec) { return n == 1024 && prec == 32 ? 4 : 2; }
+// FP32 below 1024 points: 2 and 3 are the narrow and wide plans with the
+// other tile (other_cols_z).
+inline int p3_variants(int n, int prec) { return prec == 32 ? 4 : 2; }
 
 template <typename R, int N>
-dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs) {
-  const int var = op->tune.p3_wide;
+dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, RhsFlags done, int nrhs,
+                               const dda_op_s::Tune& tn) {
+  const int var = tn.p3_wide;
   if constexpr (N == 1024 && sizeof(R) == 4) {
     if (var == 2 || var < 0) return launch_pass3_plan<R, N, Plan1024x32, 16>(op, v, st, done, nrhs);
     if (var == 3) return launch_pass3_plan<R, N, Plan1024x32, 8>(op, v, st, done, nrhs);
+  } else if constexpr (sizeof(R) == 4) {
+    if (var >= 2)
+      return (var & 1) ? launch_pass3_static_w<R, N, true, true>(op, v, st, done, nrhs)
+                       : launch_pass3_static_w<R, N, false, true>(op, v, st, done, nrhs);
   }
-  return plan_wide<R>(var) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
+  return plan_wide<R>(var >= 2 ? (var & 1) : var) ? launch_pass3_static_w<R, N, true>(op, v, st, done, nrhs)
                            : launch_pass3_static_w<R, N, false>(op, v, st, done, nrhs);
 }
 
@@ -1048,10 +1088,10 @@ dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const 
 
 template <typename R, int N>
 dda_status launch_x_static(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
-                           hipStream_t st, RhsFlags done, int nrhs) {
+                           hipStream_t st, RhsFlags done, int nrhs, const dda_op_s::Tune& tn) {
   // default narrow for both precisions: half the LDS per wave, twice the waves
   // per CU (measured 52 -> 39 us at 256^3 FP32)
-  return op->tune.x_wide[inverse] > 0
+  return tn.x_wide[inverse] > 0
              ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs)
              : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs);
 }
@@ -1136,7 +1176,7 @@ struct PassTimer {
 // inverse) of the matvec for right-hand sides r0 .. r0 + nrhs - 1 on `st`.
 template <typename R>
 static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, RhsFlags done,
-                                int r0, int nrhs, int p0, int p1) {
+                                int r0, int nrhs, int p0, int p1, const dda_op_s::Tune& tn) {
   typedef typename CxT<R>::T C;
   const MvView v = view_of(op, r0);
   const MvDims& d = v.d;
@@ -1166,7 +1206,7 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
       switch (d.Gx) {
 #define X(N)                                                                         \
   case N:                                                                            \
-    DDA_TRY((launch_x_static<R, N>(op, v, false, x, nullptr, conj, st, done, nrhs))); \
+    DDA_TRY((launch_x_static<R, N>(op, v, false, x, nullptr, conj, st, done, nrhs, tn))); \
     break;
         DDA_STATIC_SIZES(X)
 #undef X
@@ -1204,7 +1244,7 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
       RhsFlags f = col_done;
       if (f.done) f.done += (long long)q0 * f.stride;
       DDA_TRY((launch_col_any<R, false>(a1 + nb0 * cp.bstride_in, a2 + nb0 * cp.bstride_out, cp, op->plan[1],
-                                        op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[0])));
+                                        op->col_t, op->tw[1], 3 * d.Bz * k, st, f, tn.col_wide[0])));
     }
   }
   // pass 3: Z forward, D multiply, Z inverse, in place on A2
@@ -1214,7 +1254,7 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
       switch (d.Gz) {
 #define X(N)                                                         \
   case N:                                                            \
-    DDA_TRY((launch_pass3_static<R, N>(op, v, st, done, nrhs))); \
+    DDA_TRY((launch_pass3_static<R, N>(op, v, st, done, nrhs, tn))); \
     break;
         DDA_STATIC_SIZES(X)
 #undef X
@@ -1264,7 +1304,7 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
       RhsFlags f = col_done;
       if (f.done) f.done += (long long)q0 * f.stride;
       DDA_TRY((launch_col_any<R, true>(a2 + nb0 * cp.bstride_in, a1 + nb0 * cp.bstride_out, cp, op->plan[1],
-                                       op->col_t, op->tw[1], 3 * d.Bz * k, st, f, op->tune.col_wide[1])));
+                                       op->col_t, op->tw[1], 3 * d.Bz * k, st, f, tn.col_wide[1])));
     }
   }
   // pass 5
@@ -1274,7 +1314,7 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
       switch (d.Gx) {
 #define X(N)                                                                    \
   case N:                                                                       \
-    DDA_TRY((launch_x_static<R, N>(op, v, true, x, y, conj, st, done, nrhs))); \
+    DDA_TRY((launch_x_static<R, N>(op, v, true, x, y, conj, st, done, nrhs, tn))); \
     break;
         DDA_STATIC_SIZES(X)
 #undef X
@@ -1334,16 +1374,18 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   done.done = done_ptr;
   done.stride = done_stride;
   done.per = 1;
-  const bool split = (op->tune.split < 0 ? op->split_rhs : op->tune.split != 0) && nrhs >= 2 && !op->prof;
-  if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4);
+  // blocks of kBlockTuneRhs or more run with their own tuning, when there is one
+  const dda_op_s::Tune& tn = nrhs >= kBlockTuneRhs && op->tuned_blk ? op->tune_blk : op->tune;
+  const bool split = (tn.split < 0 ? op->split_rhs : tn.split != 0) && nrhs >= 2 && !op->prof;
+  if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4, tn);
   DDA_TRY(split_resources(op));
   const int h = (nrhs + 1) / 2;
   hipStream_t aux = op->split_stream;
-  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 0, 1)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 0, 1, tn)));
   DDA_TRY_HIP(hipEventRecord(op->split_ev[0], st));
   DDA_TRY_HIP(hipStreamWaitEvent(aux, op->split_ev[0], 0));
-  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 2, 4)));
-  DDA_TRY((launch_passes<R>(op, xv, yv, conj, aux, done, h, nrhs - h, 0, 4)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 2, 4, tn)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, aux, done, h, nrhs - h, 0, 4, tn)));
   DDA_TRY_HIP(hipEventRecord(op->split_ev[1], aux));
   DDA_TRY_HIP(hipStreamWaitEvent(st, op->split_ev[1], 0));
   return DDA_OK;
@@ -2370,13 +2412,13 @@ extern "C" dda_status dda_op_get_dmatrix(dda_op_t op, double* out) {
 
 namespace {
 
-typedef std::tuple<int, int, int, int, int, int, int> TuneKey;  // grid, box, precision
+typedef std::tuple<int, int, int, int, int, int, int, int> TuneKey;  // grid, box, precision, block
 std::mutex g_tune_mu;
 std::map<TuneKey, dda_op_s::Tune> g_tune_cache;
 
-TuneKey tune_key(const dda_op_s* op) {
+TuneKey tune_key(const dda_op_s* op, bool blk) {
   const MvDims& d = op->dims;
-  return TuneKey(d.Gx, d.Gy, d.Gz, d.Bx, d.By, d.Bz, op->prec);
+  return TuneKey(d.Gx, d.Gy, d.Gz, d.Bx, d.By, d.Bz, op->prec, blk ? 1 : 0);
 }
 
 // Mean time of `reps` launches of one pass (after one untimed launch).
@@ -2421,7 +2463,8 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   for (int pass = 0; pass < 5; pass++) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
-    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : 2;
+    // Z: p3_variants; Y (FP32): narrow and wide, each with the other tile too
+    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : pass != 0 && pass != 2 ? 2 : op->prec == 32 ? 4 : 2;
     // two rounds over the variants, the faster of each variant's two timings
     double t[4] = {0, 0, 0, 0};
     for (int it = 0; it < 2 * nvar; it++) {
@@ -2436,7 +2479,7 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
 #define X(N) \
   case N:    \
     return launch_x_static<R, N>(op, view_of(op, 0), pass == 4, xb, xb + 3ull * d.S * csz * nrhs, 0, st, RhsFlags(), \
-                                 nrhs);
+                                 nrhs, op->tune);
             DDA_STATIC_SIZES(X)
 #undef X
             default:
@@ -2449,7 +2492,7 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
           switch (d.Gz) {
 #define X(N) \
   case N:    \
-    return launch_pass3_static<R, N>(op, view_of(op, 0), st, RhsFlags(), nrhs);
+    return launch_pass3_static<R, N>(op, view_of(op, 0), st, RhsFlags(), nrhs, op->tune);
             DDA_STATIC_SIZES(X)
 #undef X
             default:
@@ -2495,7 +2538,9 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
     else
       best.col_wide[pass == 2] = pick;
     static const char* names[5] = {"Y forward", "Z + tensor", "Y inverse", "X forward", "X inverse"};
-    static const char* vnames[4] = {"narrow", "wide", "32x32/16 cols", "32x32/8 cols"};
+    static const char* vn1024[4] = {"narrow", "wide", "32x32/16 cols", "32x32/8 cols"};
+    static const char* vntile[4] = {"narrow", "wide", "narrow other tile", "wide other tile"};
+    const char* const* vnames = pass == 1 && d.Gz == 1024 ? vn1024 : vntile;
     std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us", report->empty() ? "" : "; ", names[pass],
                   t[0], t[1]);
     *report += buf;
@@ -2544,11 +2589,17 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
 
 }  // namespace
 
-// Applies a cached tuning for this operator's shape, if one exists.
+// Applies the cached tunings for this operator's shape (one RHS and blocks),
+// where they exist.
 void apply_cached_tuning(dda_op_s* op) {
   std::lock_guard<std::mutex> g(g_tune_mu);
-  auto it = g_tune_cache.find(tune_key(op));
+  auto it = g_tune_cache.find(tune_key(op, false));
   if (it != g_tune_cache.end()) op->tune = it->second;
+  it = g_tune_cache.find(tune_key(op, true));
+  if (it != g_tune_cache.end()) {
+    op->tune_blk = it->second;
+    op->tuned_blk = true;
+  }
 }
 
 extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len) {
@@ -2559,12 +2610,26 @@ extern "C" dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t re
   DDA_TRY(bind_device(op->ctx));
   std::string rep;
   DDA_TRY_HIP(hipStreamSynchronize(op->ctx->stream));
+  // a block of kBlockTuneRhs or more is tuned into tune_blk; while it is timed
+  // every launch runs the candidate in op->tune
+  const bool blk = nrhs >= kBlockTuneRhs;
+  const dda_op_s::Tune keep = op->tune;
+  const bool keep_blk = op->tuned_blk;
+  if (blk) {
+    if (op->tuned_blk) op->tune = op->tune_blk;
+    op->tuned_blk = false;
+  }
   dda_status rc = op->prec == 64 ? tune_impl<double>(op, nrhs, &rep) : tune_impl<float>(op, nrhs, &rep);
+  if (blk) {
+    if (rc == DDA_OK) op->tune_blk = op->tune;
+    op->tuned_blk = rc == DDA_OK || keep_blk;
+    op->tune = keep;
+  }
   if (rc != DDA_OK) return rc;
   op->tuned = true;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune_cache[tune_key(op)] = op->tune;
+    g_tune_cache[tune_key(op, blk)] = blk ? op->tune_blk : op->tune;
   }
   if (report && report_len) {
     std::strncpy(report, rep.c_str(), report_len - 1);
@@ -2590,8 +2655,8 @@ extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[6]) {
     return DDA_INVALID_ARG;
   }
   for (int i = 0; i < 6; i++)
-    if (variants[i] < -1 || variants[i] > (i == 1 ? 3 : 1)) {
-      set_error("dda_op_set_variants: variant out of range (-1 .. 1; the Z pass -1 .. 3)");
+    if (variants[i] < -1 || variants[i] > (i <= 2 ? 3 : 1)) {
+      set_error("dda_op_set_variants: variant out of range (-1 .. 3 for the Y and Z passes, -1 .. 1 otherwise)");
       return DDA_INVALID_ARG;
     }
   DDA_TRY(bind_device(op->ctx));
@@ -2604,6 +2669,7 @@ extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[6]) {
   t.x_wide[0] = variants[3];
   t.x_wide[1] = variants[4];
   t.split = variants[5];
+  op->tuned_blk = false;  // pinned variants serve every block size
   return DDA_OK;
 }
 

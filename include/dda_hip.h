@@ -178,22 +178,31 @@ dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs
 dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
 /* Autotuner (no counterpart in the reference, whose FFT plans are fixed per
  * length): times each variant of the column passes (plan width: 8 or 16
- * points per thread; for 1024-point FP32 Z columns also the 32 x 32 plan
- * over 16 or 8 columns per work-group) on this operator's own buffers for nrhs
+ * points per thread; in FP32 the Y and Z passes also with another tile, i.e.
+ * another number of columns per work-group, and 1024-point Z columns with the
+ * 32 x 32 plan over 16 or 8 columns) on this operator's own buffers for nrhs
  * right-hand sides, each twice (the faster timing counts), and keeps per pass
  * the fastest where it beats the precision's default by more than 2% (the
- * default otherwise). The choice is cached for the
- * process by (grid, box, precision), so later operators of that shape start
- * tuned, and the FP32 twin of a mixed-mode solver over a tuned operator is tuned
- * too. Results differ between variants only by rounding. `report` (nullable)
- * receives a one-line summary of the timings. */
+ * default otherwise). nrhs >= 8 tunes a separate set for blocks of 8 or
+ * more right-hand sides (matvecs of such blocks use it; smaller ones, the set
+ * of nrhs < 8). The choices are cached for the process by (grid, box,
+ * precision, block or not), so later operators of that shape start tuned. A
+ * solver over a tuned operator tunes the operator its iterations run on (the
+ * FP32 twin in the mixed and FP32 modes) for one right-hand side at creation
+ * and for blocks at its first solve of a block of 8 or more. Results differ
+ * between variants only by rounding, so with a block tuning a block's results
+ * equal its right-hand sides solved alone to rounding rather than bit for bit.
+ * `report` (nullable) receives a one-line summary of the timings. */
 dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
 /* The kernel variant of each pass, in the tuner's order: Y forward, Z with
  * the tensor product, Y inverse, X forward, X inverse; -1 = the precision's
- * default, 0 = narrow plan, 1 = wide plan; the Z pass of a 1024-point FP32
- * grid also takes 2 and 3 (the 32 x 32 plan over 16 or 8 columns). Entry 5:
+ * default, 0 = narrow plan, 1 = wide plan; in FP32 the Y and Z passes also
+ * take 2 and 3 (narrow and wide with the other tile), except the Z pass of a
+ * 1024-point grid, where 2 and 3 are the 32 x 32 plan over 16 or 8 columns.
+ * Entry 5:
  * a block of right-hand sides as two half-blocks on two streams (-1 default
- * = off, 0 off, 1 on). Setting pins the variants (e.g. a stored tuning, or a
+ * = off, 0 off, 1 on). Getting reads the set used below 8 right-hand sides;
+ * setting pins the variants for every block size (e.g. a stored tuning, or a
  * parity test of one variant); values a pass does not have run as its
  * default. No counterpart in the reference. */
 dda_status dda_op_get_variants(dda_op_t op, int variants[6]);
