@@ -507,8 +507,8 @@ __attribute__((amdgpu_waves_per_eu(P3Waves<R, N, PL::PPT>::value))) void k_pass3
 // then runs the transform from its first-stage points (only x < N/2 can hold
 // dipoles: the upper half is never read) and stores its outputs, TPC
 // consecutive points per line, coalesced.
-template <typename R, int N, bool W>
-__global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __restrict__ x,
+template <typename R, int N, bool W, int BS = 256>
+__global__ __launch_bounds__(BS) void k_pass1s(const typename CxT<R>::T* __restrict__ x,
                                                 typename CxT<R>::T* __restrict__ a1,
                                                 const uint32_t* __restrict__ row_start,
                                                 const uint32_t* __restrict__ row_dip,
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __rest
   typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
   constexpr int TPC = N / PPT;
-  constexpr int LN = 256 / TPC;
+  constexpr int LN = BS / TPC;
   constexpr int lp = (LN & 1) ? LN : LN + 1;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Out;
@@ -562,8 +562,8 @@ __global__ __launch_bounds__(256) void k_pass1s(const typename CxT<R>::T* __rest
 // Pass 5 with a static plan: lines g = c * n_list + slot over the non-empty
 // rows; inverse transform from A1 (loads coalesced), the kept half (x < N/2)
 // through the tile, then line l's dipoles gathered by its TPC threads.
-template <typename R, int N, bool W>
-__global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __restrict__ a1,
+template <typename R, int N, bool W, int BS = 256>
+__global__ __launch_bounds__(BS) void k_pass5s(const typename CxT<R>::T* __restrict__ a1,
                                                 const typename CxT<R>::T* __restrict__ x,
                                                 typename CxT<R>::T* __restrict__ y,
                                                 const uint32_t* __restrict__ row_list, int n_list,
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(256) void k_pass5s(const typename CxT<R>::T* __rest
   typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
   constexpr int TPC = N / PPT;
-  constexpr int LN = 256 / TPC;
+  constexpr int LN = BS / TPC;
   constexpr int lp = (LN & 1) ? LN : LN + 1;
   typedef typename P::Fwd F;
   typedef typename Reverse<F>::type Out;
@@ -1059,23 +1059,23 @@ dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, Rh
                            : launch_pass3_static_w<R, N, false>(op, v, st, done, nrhs);
 }
 
-template <typename R, int N, bool W>
+template <typename R, int N, bool W, int BS = 256>
 dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
                              hipStream_t st, RhsFlags done, int nrhs) {
   typedef typename CxT<R>::T C;
   constexpr int TPC = N / StaticPlan<N, W>::PPT;
-  constexpr int LN = 256 / TPC;
+  constexpr int LN = BS / TPC;
   constexpr int lp = (LN & 1) ? LN : LN + 1;
   const MvDims& d = v.d;
   const size_t lds = (size_t)N * lp * sizeof(C);
   if (!inverse) {
-    auto k = k_pass1s<R, N, W>;
+    auto k = k_pass1s<R, N, W, BS>;
     DDA_TRY(set_lds(k, lds));
     hipLaunchKernelGGL(k, dim3((3 * d.nrows + LN - 1) / LN, nrhs), dim3(LN * TPC), lds, st, (const C*)x, (C*)v.a1,
                        op->row_start, op->row_dip, op->row_meta, (const C*)v.ccs, d, (const C*)op->tw[0], conj,
                        done);
   } else {
-    auto k = k_pass5s<R, N, W>;
+    auto k = k_pass5s<R, N, W, BS>;
     DDA_TRY(set_lds(k, lds));
     const int nl = 3 * op->n_nonempty;
     hipLaunchKernelGGL(k, dim3(nl > 0 ? (nl + LN - 1) / LN : 1, nrhs), dim3(LN * TPC), lds, st, (const C*)v.a1,
@@ -1090,10 +1090,16 @@ template <typename R, int N>
 dda_status launch_x_static(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
                            hipStream_t st, RhsFlags done, int nrhs, const dda_op_s::Tune& tn) {
   // default narrow for both precisions: half the LDS per wave, twice the waves
-  // per CU (measured 52 -> 39 us at 256^3 FP32)
-  return tn.x_wide[inverse] > 0
-             ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs)
-             : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs);
+  // per CU (measured 52 -> 39 us at 256^3 FP32); FP32 also 2 and 3, narrow
+  // and wide over 512-thread work-groups (twice the lines each)
+  const int var = tn.x_wide[inverse];
+  if constexpr (sizeof(R) == 4) {
+    if (var >= 2)
+      return (var & 1) ? launch_x_static_w<R, N, true, 512>(op, v, inverse, x, y, conj, st, done, nrhs)
+                       : launch_x_static_w<R, N, false, 512>(op, v, inverse, x, y, conj, st, done, nrhs);
+  }
+  return var >= 0 && (var & 1) ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs)
+                               : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs);
 }
 
 // Lengths with a static plan; others use the run-time planned kernels.
@@ -2463,8 +2469,8 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   for (int pass = 0; pass < 5; pass++) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
-    // Z: p3_variants; Y (FP32): narrow and wide, each with the other tile too
-    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : pass != 0 && pass != 2 ? 2 : op->prec == 32 ? 4 : 2;
+    // Z: p3_variants; Y and X (FP32): narrow and wide, each with the other tile too
+    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : op->prec == 32 ? 4 : 2;
     // two rounds over the variants, the faster of each variant's two timings
     double t[4] = {0, 0, 0, 0};
     for (int it = 0; it < 2 * nvar; it++) {
@@ -2526,7 +2532,8 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
     }
     // the precision's default unless another variant is faster by more than
     // 2% (closer timings change places between runs)
-    const int def = pass == 1 && d.Gz == 1024 && sizeof(R) == 4 ? 2 : sizeof(R) == 4 ? 1 : 0;
+    // (the X passes default to the narrow plan in both precisions)
+    const int def = pass == 1 && d.Gz == 1024 && sizeof(R) == 4 ? 2 : pass < 3 && sizeof(R) == 4 ? 1 : 0;
     int fastest = 0;
     for (int w = 1; w < nvar; w++)
       if (t[w] < t[fastest]) fastest = w;
@@ -2655,8 +2662,8 @@ extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[6]) {
     return DDA_INVALID_ARG;
   }
   for (int i = 0; i < 6; i++)
-    if (variants[i] < -1 || variants[i] > (i <= 2 ? 3 : 1)) {
-      set_error("dda_op_set_variants: variant out of range (-1 .. 3 for the Y and Z passes, -1 .. 1 otherwise)");
+    if (variants[i] < -1 || variants[i] > (i <= 4 ? 3 : 1)) {
+      set_error("dda_op_set_variants: variant out of range (-1 .. 3 for the passes, -1 .. 1 for the split)");
       return DDA_INVALID_ARG;
     }
   DDA_TRY(bind_device(op->ctx));

@@ -178,9 +178,9 @@ dda_status dda_op_set_cc_sqrt_batch(dda_op_t op, const double* cc_sqrt, int nrhs
 dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int nrhs, int conj, dda_stream_t stream);
 /* Autotuner (no counterpart in the reference, whose FFT plans are fixed per
  * length): times each variant of the column passes (plan width: 8 or 16
- * points per thread; in FP32 the Y and Z passes also with another tile, i.e.
- * another number of columns per work-group, and 1024-point Z columns with the
- * 32 x 32 plan over 16 or 8 columns) on this operator's own buffers for nrhs
+ * points per thread; in FP32 every pass also with another tile, i.e.
+ * another number of columns or lines per work-group, and 1024-point Z columns
+ * with the 32 x 32 plan over 16 or 8 columns) on this operator's own buffers for nrhs
  * right-hand sides, each twice (the faster timing counts), and keeps per pass
  * the fastest where it beats the precision's default by more than 2% (the
  * default otherwise). nrhs >= 8 tunes a separate set for blocks of 8 or
@@ -196,8 +196,8 @@ dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int n
 dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
 /* The kernel variant of each pass, in the tuner's order: Y forward, Z with
  * the tensor product, Y inverse, X forward, X inverse; -1 = the precision's
- * default, 0 = narrow plan, 1 = wide plan; in FP32 the Y and Z passes also
- * take 2 and 3 (narrow and wide with the other tile), except the Z pass of a
+ * default, 0 = narrow plan, 1 = wide plan; in FP32 every pass also takes 2
+ * and 3 (narrow and wide with the other tile), except the Z pass of a
  * 1024-point grid, where 2 and 3 are the 32 x 32 plan over 16 or 8 columns.
  * Entry 5:
  * a block of right-hand sides as two half-blocks on two streams (-1 default

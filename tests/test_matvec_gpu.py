@@ -130,10 +130,11 @@ def test_rows_without_dipoles_stay_zero(ctx, name, grid, precision):
 
 
 @pytest.mark.parametrize("grid", [(32, 64, 96), (32, 96, 64), (32, 128, 192), (32, 192, 128), (32, 256, 512),
-                                  (32, 512, 256), (32, 1024, 16), (32, 16, 1024), (32, 48, 32)])
+                                  (32, 512, 256), (32, 1024, 16), (32, 16, 1024), (32, 48, 32), (96, 16, 16),
+                                  (256, 16, 16), (1024, 16, 16)])
 @pytest.mark.parametrize("variant", [2, 3])
 def test_matvec_other_tile_variants(ctx, grid, variant):
-    """The FP32 Y and Z passes with the other tile (more or fewer columns per
+    """Every FP32 pass with the other tile (more or fewer columns or lines per
     work-group; at 1024 Z points the 32 x 32 plans), narrow (2) and wide (3),
     pinned through dda_op_set_variants, against the oracle."""
     print("grid", grid, "variant", variant, flush=True)
@@ -142,7 +143,7 @@ def test_matvec_other_tile_variants(ctx, grid, variant):
     orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
     x = P.random_dipole_vector(params, seed=14)
     op = _ops(ctx, g, params, cs, 32)
-    assert op.variants([variant, variant, variant, -1, -1, -1])[:3] == [variant] * 3
+    assert op.variants([variant] * 5 + [-1])[:5] == [variant] * 5
     for conj in (False, True):
         y = D.MatVecGpu(op).apply(x, conj=conj)
         assert cases.rel_err(y, orc.matvec(x, conj=conj, precision=64)) <= FP32_TOL
@@ -157,7 +158,7 @@ def test_variants_rejects_out_of_range(ctx):
     with pytest.raises(D.DDAError):
         op.variants([4, -1, -1, -1, -1, -1])
     with pytest.raises(D.DDAError):
-        op.variants([-1, -1, -1, 2, -1, -1])
+        op.variants([-1, -1, -1, 4, -1, -1])
     with pytest.raises(D.DDAError):
         op.variants([-1, -1, -1, -1, -1, 2])
     assert op.variants() == [-1, -1, -1, -1, -1, -1]
