@@ -125,7 +125,7 @@ def matvec_leg(ctx, op64, g, params, rhs, D, P, np):
     op32 = D.GpuMatVec(ctx, g.pos, g.mat, g.box, g.grid_unit, params.wave_num, precision=32, grid=params.grid)
     try:
         op32.set_couple_constants_batch(np.array([r[2] for r in rhs]))
-        op32.tune(2)
+        out["fp32_tuning"] = op32.tune(2)
         x1 = P.random_dipole_vector(params)
         x2 = np.concatenate([x1, P.random_dipole_vector(params, seed=0x5EED + 1)])
         for name, op, prec, nrhs in (("fp64_1rhs", op64, 64, 1), ("fp32_1rhs", op32, 32, 1),
