@@ -388,8 +388,8 @@ class GpuMatVec:
         _check(lib().dda_op_apply_batch(self.handle, xp, yp, int(nrhs), int(conj), stream), "dda_op_apply_batch")
 
     def tune(self, nrhs=1):
-        """Autotune the kernel variants for this shape (cached per process); nrhs >= 8 tunes
-        the separate set used by blocks of 8 or more right-hand sides. Returns the timing summary."""
+        """Autotune the kernel variants for this shape (cached per process); nrhs >= 2 tunes
+        the separate set used by blocks of right-hand sides. Returns the timing summary."""
         buf = ctypes.create_string_buffer(512)
         _check(lib().dda_op_tune(self.handle, int(nrhs), buf, 512), "dda_op_tune")
         return buf.value.decode()

@@ -84,7 +84,7 @@ dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj
 // device, later calls return at once. dda_ctx_create calls it.
 constexpr int kMaxNoAllocDevices = 64;
 // Blocks of at least this many right-hand sides have a tuning of their own.
-constexpr int kBlockTuneRhs = 8;
+constexpr int kBlockTuneRhs = 2;
 dda_status noalloc_scratch_setup(int device);
 
 }  // namespace dda

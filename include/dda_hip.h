@@ -183,13 +183,13 @@ dda_status dda_op_apply_batch(dda_op_t op, const void* x_dev, void* y_dev, int n
  * with the 32 x 32 plan over 16 or 8 columns) on this operator's own buffers for nrhs
  * right-hand sides, each twice (the faster timing counts), and keeps per pass
  * the fastest where it beats the precision's default by more than 2% (the
- * default otherwise). nrhs >= 8 tunes a separate set for blocks of 8 or
- * more right-hand sides (matvecs of such blocks use it; smaller ones, the set
- * of nrhs < 8). The choices are cached for the process by (grid, box,
+ * default otherwise). nrhs >= 2 tunes a separate set for blocks of right-hand
+ * sides (matvecs of blocks use it; single right-hand sides, the set of
+ * nrhs = 1). The choices are cached for the process by (grid, box,
  * precision, block or not), so later operators of that shape start tuned. A
  * solver over a tuned operator tunes the operator its iterations run on (the
  * FP32 twin in the mixed and FP32 modes) for one right-hand side at creation
- * and for blocks at its first solve of a block of 8 or more. Results differ
+ * and for blocks at its first block solve. Results differ
  * between variants only by rounding, so with a block tuning a block's results
  * equal its right-hand sides solved alone to rounding rather than bit for bit.
  * `report` (nullable) receives a one-line summary of the timings. */
@@ -201,7 +201,7 @@ dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
  * 1024-point grid, where 2 and 3 are the 32 x 32 plan over 16 or 8 columns.
  * Entry 5:
  * a block of right-hand sides as two half-blocks on two streams (-1 default
- * = off, 0 off, 1 on). Getting reads the set used below 8 right-hand sides;
+ * = off, 0 off, 1 on). Getting reads the set used for single right-hand sides;
  * setting pins the variants for every block size (e.g. a stored tuning, or a
  * parity test of one variant); values a pass does not have run as its
  * default. No counterpart in the reference. */

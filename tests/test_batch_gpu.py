@@ -216,10 +216,10 @@ def test_split_block_matches_one_stream(ctx, precision, mode):
 
 
 def test_block_tuning_is_separate_and_pinning_covers_every_block(ctx):
-    """dda_op_tune with 8 or more right-hand sides tunes a set of its own for
-    blocks: the set below 8 is unchanged, a tuned block's every result matches
-    the oracle, and pinned variants serve every block size again (a block of 8
-    then gives bitwise each right-hand side's result alone)."""
+    """dda_op_tune with 2 or more right-hand sides tunes a set of its own for
+    blocks: the set for single right-hand sides is unchanged, a tuned block's
+    every result matches the oracle, and pinned variants serve every block size
+    again (a block of 8 then gives bitwise each right-hand side's result alone)."""
     g, params, einc, ccs = _setup("ellipsoid_5_4_3", 4, 12, (32, 64, 96))
     k, n = len(ccs), params.vec_size
     assert k == 8
@@ -254,7 +254,7 @@ def test_block_tuning_is_separate_and_pinning_covers_every_block(ctx):
 
 def test_mixed_block_solve_over_tuned_operator(ctx):
     """A mixed-mode block of 8 over a tuned operator (its FP32 twin then tuned
-    for blocks at the first solve) converges and matches the same block over
+    for blocks at the first block solve) converges and matches the same block over
     an untuned operator to the solver's tolerance."""
     g, params, einc, ccs = _setup("ellipsoid_5_4_3", 4, 13, (32, 64, 96))
     k = len(ccs)
