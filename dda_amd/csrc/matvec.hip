@@ -916,13 +916,21 @@ template <int T, int TPC>
 constexpr int other_cols() {
   return 2 * T * TPC <= 1024 ? 2 * T : (T / 2) * TPC >= 64 ? T / 2 : T;
 }
+// and the third: four times the default columns where that stays within 1024
+// threads and the LDS (FP32 tiles of N x (4T + 1) points), else the second
+template <int T, int TPC, int N>
+constexpr int third_cols() {
+  return 4 * T * TPC <= 1024 && (size_t)N * (4 * T + 1) * 8 <= 160 * 1024 ? 4 * T : other_cols<T, TPC>();
+}
 
-template <typename R, int N, bool W, bool INV, bool ALT = false>
+// ALT: 0 the default tile, 1 other_cols, 2 third_cols
+template <typename R, int N, bool W, bool INV, int ALT = 0>
 dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch,
                                hipStream_t st, RhsFlags done) {
   typedef typename CxT<R>::T C;
   constexpr int T0 = static_cols<R, N, W, 1024>();
-  constexpr int T = ALT ? other_cols<T0, N / StaticPlan<N, W>::PPT>() : T0;
+  constexpr int TPC0 = N / StaticPlan<N, W>::PPT;
+  constexpr int T = ALT == 0 ? T0 : ALT == 1 ? other_cols<T0, TPC0>() : third_cols<T0, TPC0, N>();
   constexpr int threads = T * (N / StaticPlan<N, W>::PPT);
   constexpr int lp = (T & 1) ? T : T + 1;
   const size_t lds = (size_t)N * lp * sizeof(C);
@@ -930,9 +938,9 @@ dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, c
   // keeps at most N/2; the tensor generation transforms full lengths
   const bool hin = !INV && 2 * cp.in_len <= N, hout = INV && 2 * cp.out_len <= N;
   void (*k)(const C*, C*, ColParams, const C*, RhsFlags);
-  if constexpr (ALT) {
-    // the other tile is a matvec variant: full-length transforms keep the default
-    if (!hin && !hout) return launch_col_static_w<R, N, W, INV, false>(in, out, cp, tw, nbatch, st, done);
+  if constexpr (ALT != 0) {
+    // the other tiles are matvec variants: full-length transforms keep the default
+    if (!hin && !hout) return launch_col_static_w<R, N, W, INV, 0>(in, out, cp, tw, nbatch, st, done);
     k = INV ? k_colfft_s<R, N, T, W, INV, false, INV> : k_colfft_s<R, N, T, W, INV, !INV, false>;
   } else {
     k = hin ? k_colfft_s<R, N, T, W, INV, !INV, false>
@@ -946,14 +954,18 @@ dda_status launch_col_static_w(const void* in, void* out, const ColParams& cp, c
 }
 
 // wide: -1 the precision's default plan, 0 narrow, 1 wide; FP32 also 2 and 3,
-// narrow and wide with the other tile (other_cols).
+// narrow and wide with the second tile (other_cols), and 4 and 5 with the
+// third (third_cols).
 template <typename R, int N, bool INV>
 dda_status launch_col_static(const void* in, void* out, const ColParams& cp, const void* tw, int nbatch, hipStream_t st,
                              RhsFlags done, int wide) {
   if constexpr (sizeof(R) == 4) {
+    if (wide >= 4)
+      return (wide & 1) ? launch_col_static_w<R, N, true, INV, 2>(in, out, cp, tw, nbatch, st, done)
+                        : launch_col_static_w<R, N, false, INV, 2>(in, out, cp, tw, nbatch, st, done);
     if (wide >= 2)
-      return (wide & 1) ? launch_col_static_w<R, N, true, INV, true>(in, out, cp, tw, nbatch, st, done)
-                        : launch_col_static_w<R, N, false, INV, true>(in, out, cp, tw, nbatch, st, done);
+      return (wide & 1) ? launch_col_static_w<R, N, true, INV, 1>(in, out, cp, tw, nbatch, st, done)
+                        : launch_col_static_w<R, N, false, INV, 1>(in, out, cp, tw, nbatch, st, done);
   }
   return plan_wide<R>(wide >= 2 ? (wide & 1) : wide) ? launch_col_static_w<R, N, true, INV>(in, out, cp, tw, nbatch, st, done)
                                                     : launch_col_static_w<R, N, false, INV>(in, out, cp, tw, nbatch, st, done);
@@ -2469,10 +2481,11 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
   for (int pass = 0; pass < 5; pass++) {
     const bool is_static = pass == 1 ? op->static_z : pass >= 3 ? op->static_x : has_static_plan(d.Gy);
     if (!is_static) continue;
-    // Z: p3_variants; Y and X (FP32): narrow and wide, each with the other tile too
-    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : op->prec == 32 ? 4 : 2;
+    // Z: p3_variants; FP32 X: narrow and wide, each with the other tile too;
+    // FP32 Y: with the second and the third tile
+    const int nvar = pass == 1 ? p3_variants(d.Gz, op->prec) : op->prec != 32 ? 2 : pass >= 3 ? 4 : 6;
     // two rounds over the variants, the faster of each variant's two timings
-    double t[4] = {0, 0, 0, 0};
+    double t[6] = {0, 0, 0, 0, 0, 0};
     for (int it = 0; it < 2 * nvar; it++) {
       const int w = it % nvar;
       double tt = 0;
@@ -2546,7 +2559,8 @@ dda_status tune_impl(dda_op_s* op, int nrhs, std::string* report) {
       best.col_wide[pass == 2] = pick;
     static const char* names[5] = {"Y forward", "Z + tensor", "Y inverse", "X forward", "X inverse"};
     static const char* vn1024[4] = {"narrow", "wide", "32x32/16 cols", "32x32/8 cols"};
-    static const char* vntile[4] = {"narrow", "wide", "narrow other tile", "wide other tile"};
+    static const char* vntile[6] = {"narrow", "wide", "narrow other tile", "wide other tile", "narrow third tile",
+                                    "wide third tile"};
     const char* const* vnames = pass == 1 && d.Gz == 1024 ? vn1024 : vntile;
     std::snprintf(buf, sizeof buf, "%s%s: narrow %.1f us, wide %.1f us", report->empty() ? "" : "; ", names[pass],
                   t[0], t[1]);
@@ -2662,8 +2676,9 @@ extern "C" dda_status dda_op_set_variants(dda_op_t op, const int variants[6]) {
     return DDA_INVALID_ARG;
   }
   for (int i = 0; i < 6; i++)
-    if (variants[i] < -1 || variants[i] > (i <= 4 ? 3 : 1)) {
-      set_error("dda_op_set_variants: variant out of range (-1 .. 3 for the passes, -1 .. 1 for the split)");
+    if (variants[i] < -1 || variants[i] > (i == 0 || i == 2 ? 5 : i <= 4 ? 3 : 1)) {
+      set_error("dda_op_set_variants: variant out of range (-1 .. 5 for the Y passes, -1 .. 3 for the others, "
+                "-1 .. 1 for the split)");
       return DDA_INVALID_ARG;
     }
   DDA_TRY(bind_device(op->ctx));

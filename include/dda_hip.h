@@ -198,7 +198,9 @@ dda_status dda_op_tune(dda_op_t op, int nrhs, char* report, size_t report_len);
  * the tensor product, Y inverse, X forward, X inverse; -1 = the precision's
  * default, 0 = narrow plan, 1 = wide plan; in FP32 every pass also takes 2
  * and 3 (narrow and wide with the other tile), except the Z pass of a
- * 1024-point grid, where 2 and 3 are the 32 x 32 plan over 16 or 8 columns.
+ * 1024-point grid, where 2 and 3 are the 32 x 32 plan over 16 or 8 columns;
+ * the Y passes also 4 and 5 (a third tile, four times the default columns
+ * where they fit).
  * Entry 5:
  * a block of right-hand sides as two half-blocks on two streams (-1 default
  * = off, 0 off, 1 on). Getting reads the set used for single right-hand sides;

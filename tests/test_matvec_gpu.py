@@ -150,13 +150,32 @@ def test_matvec_other_tile_variants(ctx, grid, variant):
     op.close()
 
 
+@pytest.mark.parametrize("grid", [(32, 64, 96), (32, 96, 64), (32, 128, 192), (32, 192, 128), (32, 256, 512),
+                                  (32, 512, 256), (32, 1024, 16), (32, 48, 32)])
+@pytest.mark.parametrize("variant", [4, 5])
+def test_matvec_third_tile_y_variants(ctx, grid, variant):
+    """The FP32 Y passes with the third tile (four times the default columns
+    where they fit), narrow (4) and wide (5), against the oracle."""
+    print("grid", grid, "variant", variant, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("ellipsoid_5_4_3", grid=grid)
+    assert params.grid == grid
+    orc = cases.oracle_problem(g, params, cs, dmat_precision=64)
+    x = P.random_dipole_vector(params, seed=15)
+    op = _ops(ctx, g, params, cs, 32)
+    assert op.variants([variant, -1, variant, -1, -1, -1])[0] == variant
+    for conj in (False, True):
+        y = D.MatVecGpu(op).apply(x, conj=conj)
+        assert cases.rel_err(y, orc.matvec(x, conj=conj, precision=64)) <= FP32_TOL
+    op.close()
+
+
 def test_variants_rejects_out_of_range(ctx):
     g, params, cc, cs, prop, pol = cases.build("sphere_r4")
     op = _ops(ctx, g, params, cs, 32)
     with pytest.raises(D.DDAError):
         op.variants([-1, 4, -1, -1, -1, -1])
     with pytest.raises(D.DDAError):
-        op.variants([4, -1, -1, -1, -1, -1])
+        op.variants([6, -1, -1, -1, -1, -1])
     with pytest.raises(D.DDAError):
         op.variants([-1, -1, -1, 4, -1, -1])
     with pytest.raises(D.DDAError):
