@@ -223,11 +223,12 @@ def c3_bench(args, D, P, np, ctx):
     return out
 
 
-def large_matvec(args, D, P, np, ctx, solves=False):
+def large_matvec(args, D, P, np, ctx, solvers=()):
     """The largest single-GPU grid: sphere 512 dipoles across (~70M dipoles,
     1024^3), m = 1.6+0.05i, |m| kd = 0.5. FP32 and FP64 matvec rates against
-    B_mv, the device memory held, and optionally mixed BiCGStab / QMR-CS to
-    1e-5 (one polarisation each)."""
+    B_mv, the device memory held, and mixed solves to 1e-5 with the given
+    solvers ("bicgstab", "qmr"; one polarisation each), their Cext and Cabs
+    beside Mie theory for the volume-equivalent sphere."""
     m = 1.6 + 0.05j
     d = 1e-7
     kd = 0.5 / abs(m)
@@ -283,10 +284,19 @@ def large_matvec(args, D, P, np, ctx, solves=False):
             op64 = op
         else:
             op.close()
-    if solves:
+    if solvers:
         einc = P.einc_plane(params, prop, pol1)
         out["solves"] = {}
-        for kind in args.large_solvers.split(","):
+        mie = None
+        try:
+            import oracle as O  # the checker of the cross sections only
+            a_eq = d * (3.0 * params.nv / (4.0 * math.pi)) ** (1.0 / 3.0)
+            qext, qsca, _, _ = O.mie(params.wave_num * a_eq, m)
+            area = math.pi * a_eq * a_eq
+            mie = (qext * area, (qext - qsca) * area)
+        except Exception as e:
+            out["mie_error"] = str(e)
+        for kind in solvers:
             s = D.SOLVER_CLASSES[kind](op64, mixed=True, max_iter=args.large_max_iter)
             t0 = time.perf_counter()
             try:
@@ -300,6 +310,10 @@ def large_matvec(args, D, P, np, ctx, solves=False):
                    "refinements": info["refinements"], "true_rel_residual": info["true_rel_residual"]}
             if p is not None:
                 res["cext"] = P.ext_cross(params, p, einc)
+                res["cabs"] = P.abs_cross(params, p, cc)
+                if mie is not None:
+                    res["cext_vs_mie"] = res["cext"] / mie[0] - 1
+                    res["cabs_vs_mie"] = res["cabs"] / mie[1] - 1
             out["solves"][kind] = res
             print("large solve %s: %s" % (kind, json.dumps(res)), file=sys.stderr, flush=True)
             s.close()
@@ -310,7 +324,8 @@ def large_matvec(args, D, P, np, ctx, solves=False):
 
 def large_bench(args, D, P, np):
     ctx = D.Context(0)
-    print(json.dumps(large_matvec(args, D, P, np, ctx, solves=args.large_solves)))
+    kinds = args.large_solvers.split(",") if args.large_solves else ()
+    print(json.dumps(large_matvec(args, D, P, np, ctx, solvers=kinds)))
     return 0
 
 
@@ -427,6 +442,8 @@ def main():
                     help="only the serving measurement (one JSON line); the full run starts itself this way")
     ap.add_argument("--no-large-grid", action="store_true",
                     help="skip the 1024^3 (sphere 512 across) FP32 / FP64 matvec figures")
+    ap.add_argument("--no-large-solves", action="store_true",
+                    help="skip the mixed BiCGStab solve to 1e-5 on the 1024^3 grid (about a minute)")
     ap.add_argument("--matvec-profile", type=int, default=0, metavar="K",
                     help="instead of solves: one 512 MiB device copy (counter calibration), then K FP32 and "
                          "K FP64 matvecs; prints per-pass times (for rocprofv3 counter runs)")
@@ -757,7 +774,7 @@ def main():
     if not args.no_large_grid:
         try:
             op.close()  # the headline operator's memory back before the 1024^3 operators
-            large = large_matvec(args, D, P, np, ctx)
+            large = large_matvec(args, D, P, np, ctx, solvers=() if args.no_large_solves else ("bicgstab",))
         except Exception as e:  # reported, not fatal
             large = {"error": str(e)}
 
