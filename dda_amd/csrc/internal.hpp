@@ -72,12 +72,26 @@ dda_status create_twin32(const struct ::dda_op_s* op, struct ::dda_op_s** out);
 bool last_tensor_cache_hit();
 double tensor_cache_seconds();
 
+// Sums that pass 5 can form, per work-group, over the vector x it gathers and
+// the y it writes (both as stored), for a solver's next scalar step: x.y
+// (DOT_U: 2 quantities), |y|^2 (DOT_NORM: 1), |y|^2 and x.conj(y) (DOT_OM: 3).
+enum PassDot { DOT_NONE = 0, DOT_U = 1, DOT_NORM = 2, DOT_OM = 3 };
+struct PassDotReq {
+  double* part = nullptr;  // quantity q of work-group b of right-hand side r at part[(r * 4 + q) * nb + b]
+  int op = DOT_NONE;
+  int r0 = 0;              // right-hand side of the launch's first block row
+};
+
 // Launch helpers implemented in matvec.hip.
 // nrhs right-hand sides stored back to back ([nrhs][3][S]); done (nullable)
-// points at the first RHS's stop flag, done_stride ints apart.
+// points at the first RHS's stop flag, done_stride ints apart. dot_part /
+// dot_op (compile-time planned X passes only): pass 5 also writes the partial
+// sums dot_op of x and y to dot_part, nb = op->p5_blocks work-groups per
+// right-hand side.
 template <typename R>
 dda_status launch_matvec(struct ::dda_op_s* op, const void* x, void* y, int conj, hipStream_t st,
-                         const int* done, int nrhs = 1, int done_stride = 0);
+                         const int* done, int nrhs = 1, int done_stride = 0, double* dot_part = nullptr,
+                         int dot_op = 0);
 
 // Takes the device address of the fixed scratch array of the allocation-free
 // entry point (observe.hip) for `device`, which must be current; once per
@@ -147,6 +161,7 @@ struct dda_op_s {
 
   // launch shapes
   int p1_rows = 1;   // rows per work-group in the X passes
+  int p5_blocks = 0; // work-groups per right-hand side of the last static pass 5 launched
   int col_t = 8;     // columns per work-group in the Y/Z passes
   // Kernel variants (plan width of the static column passes: -1 default,
   // 0 narrow, 1 wide; the Z pass at 1024 points also 2, 3: the 32 x 32 plan

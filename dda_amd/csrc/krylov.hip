@@ -861,6 +861,10 @@ struct Core {
   SolverState* dst = nullptr;       // device [cap]
   SolverState* host2 = nullptr;     // pinned [2][cap]
   double* part = nullptr;           // [cap][4][nb]
+  // [cap][4][at most 3 * n_nonempty]: partial sums formed by pass 5 of a
+  // matvec (compile-time planned X passes), nb = op->p5_blocks
+  double* part5 = nullptr;
+  bool fused = false;               // this solve takes those sums from pass 5
   double* hist = nullptr;           // [cap][hist_cap]
   uint64_t hist_cap = 0;
   uint64_t max_iter = 0, max_res_increase = 0;
@@ -883,6 +887,26 @@ struct Core {
   }
   void scalar(int step) {
     hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kRedThreads), 0, st, dst, part, nb, step, kind, hist);
+  }
+  // y = A x, then scalar step `step` over the partial sums `dop` of x and y:
+  // formed by the matvec's pass 5 when fused, else by a reduction kernel
+  // (DOT_U: k_reduce DOTU; DOT_NORM: k_reduce NORM of y; DOT_OM: k_bs_om).
+  dda_status matvec_sums(const C* x, C* y, int conj, int ioff, int dop, int step) {
+    if (fused) {
+      DDA_TRY(launch_matvec<R>(op, x, y, conj, st, flag(ioff), nrhs, kStateInts, part5, dop));
+      hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kRedThreads), 0, st, dst, part5, op->p5_blocks, step, kind,
+                         hist);
+      return DDA_OK;
+    }
+    DDA_TRY(matvec(x, y, conj, ioff));
+    if (dop == DOT_U)
+      reduce(RED_DOTU, x, y, ioff);
+    else if (dop == DOT_NORM)
+      reduce(RED_NORM, y, y, ioff);
+    else
+      hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, y, x, n, part, dst);
+    scalar(step);
+    return DDA_OK;
   }
   void reduce(int opk, const C* a, const C* bb, int ioff) {
     if (opk == RED_NORM)
@@ -910,6 +934,8 @@ struct Core {
     DDA_TRY_HIP(hipMalloc(&dst, cap * sizeof(SolverState)));
     DDA_TRY_HIP(hipHostMalloc((void**)&host2, 2 * cap * sizeof(SolverState), hipHostMallocDefault));
     DDA_TRY_HIP(hipMalloc(&part, (size_t)cap * 4 * nb * sizeof(double)));
+    if (o->static_x)
+      DDA_TRY_HIP(hipMalloc(&part5, (size_t)cap * 4 * std::max(1, 3 * o->n_nonempty) * sizeof(double)));
     DDA_TRY_HIP(hipMalloc(&hist, (size_t)cap * hist_cap * sizeof(double)));
     for (int i = 0; i < 8; i++) DDA_TRY_HIP(hipMalloc(&v[i], (size_t)cap * n * sizeof(C)));
     return DDA_OK;
@@ -922,8 +948,9 @@ struct Core {
   std::vector<const void*> graph_signature() const {
     // every buffer a captured launch references that can be reallocated
     // (a larger block of right-hand sides, a pack's member table)
-    std::vector<const void*> g = {dst, part, hist, op->a1, op->a2, op->ccs, op->dhat, op->dims.member,
-                                  op->dims.dhat_tab, reinterpret_cast<const void*>((intptr_t)nrhs)};
+    std::vector<const void*> g = {dst, part, part5, hist, op->a1, op->a2, op->ccs, op->dhat, op->dims.member,
+                                  op->dims.dhat_tab, reinterpret_cast<const void*>((intptr_t)nrhs),
+                                  reinterpret_cast<const void*>((intptr_t)fused)};
     for (const dda_op_s::Tune* tn : {&op->tune, &op->tune_blk})
       for (int t : {tn->col_wide[0], tn->col_wide[1], tn->p3_wide, tn->x_wide[0], tn->x_wide[1], tn->split})
         g.push_back(reinterpret_cast<const void*>((intptr_t)t));
@@ -936,12 +963,14 @@ struct Core {
     if (dst) (void)hipFree(dst);
     if (host2) (void)hipHostFree(host2);
     if (part) (void)hipFree(part);
+    if (part5) (void)hipFree(part5);
     if (hist) (void)hipFree(hist);
     for (C* p : v)
       if (p) (void)hipFree(p);
     dst = nullptr;
     host2 = nullptr;
     part = nullptr;
+    part5 = nullptr;
     hist = nullptr;
     for (auto& p : v) p = nullptr;
     cap = 0;
@@ -958,6 +987,9 @@ struct Core {
   dda_status solve(int k, const double* eps, const int* active, bool have_x0, SolverState* fin,
                    const uint64_t* max_iters = nullptr, double* profile_us = nullptr, int profile_n = 0) {
     nrhs = k;
+    // the sums after a matvec come from its pass 5 where the X passes have
+    // a compile-time plan (DDA_NO_FUSED_SUMS=1: from reduction kernels)
+    fused = part5 && !std::getenv("DDA_NO_FUSED_SUMS");
     C* bv = v[0];
     C* r = v[1];
     C* x = v[2];
@@ -1008,9 +1040,7 @@ struct Core {
         case DDA_SOLVER_QMR_CS: {
           C* vv = v[iv];
           C* vt = v[ivt];
-          DDA_TRY(matvec(vv, av, 0, DONE));
-          reduce(RED_DOTU, vv, av, DONE);
-          scalar(S_QMR_ALPHA);
+          DDA_TRY(matvec_sums(vv, av, 0, DONE, DOT_U, S_QMR_ALPHA));  // alpha = v.Av
           hipLaunchKernelGGL(k_qmr_vt<R>, g(), b(), 0, st, vt, vv, av, nr == 0 ? 1 : 0, dst, n, part);
           scalar(S_QMR_GIVENS);
           const int mode = nr == 0 ? 0 : (nr == 1 ? 1 : 2);
@@ -1037,22 +1067,16 @@ struct Core {
           scalar(S_BS_ALPHA);
           lincomb(sv, vv, DDA_OFF(mAlpha), nullptr, -1, r, true, DONE);  // s = v (-alpha) + r
           scalar(S_BS_S);
-          DDA_TRY(matvec(sv, av, 0, SKIP));
-          hipLaunchKernelGGL(k_bs_om<R>, g(), b(), 0, st, av, sv, n, part, dst);
-          scalar(S_BS_OMEGA);
+          DDA_TRY(matvec_sums(sv, av, 0, SKIP, DOT_OM, S_BS_OMEGA));  // t = A s; |t|^2, s.conj(t)
           hipLaunchKernelGGL(k_bs_upd<R>, g(), b(), 0, st, x, r, p, sv, av, rt, n, part, dst);
           scalar(S_END_ITER);
           break;
         }
         case DDA_SOLVER_CGNR: {
           C* p = v[4];
-          DDA_TRY(matvec(r, av, 1, DONE));  // A^H r
-          reduce(RED_NORM, av, av, DONE);
-          scalar(S_CG_RO);
+          DDA_TRY(matvec_sums(r, av, 1, DONE, DOT_NORM, S_CG_RO));  // A^H r, |A^H r|^2
           hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, av, nr == 0 ? 1 : 0, dst, n);
-          DDA_TRY(matvec(p, av, 0, DONE));
-          reduce(RED_NORM, av, av, DONE);
-          scalar(S_CG_ALPHA);
+          DDA_TRY(matvec_sums(p, av, 0, DONE, DOT_NORM, S_CG_ALPHA));  // |A p|^2
           hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
           scalar(S_END_ITER);
           break;
@@ -1062,9 +1086,7 @@ struct Core {
           reduce(RED_DOTU, r, r, DONE);
           scalar(S_BI_RO);
           hipLaunchKernelGGL(k_p_update<R>, g(), b(), 0, st, p, r, nr == 0 ? 1 : 0, dst, n);
-          DDA_TRY(matvec(p, av, 0, DONE));
-          reduce(RED_DOTU, p, av, DONE);
-          scalar(S_BI_MU);
+          DDA_TRY(matvec_sums(p, av, 0, DONE, DOT_U, S_BI_MU));  // mu = p.Ap
           hipLaunchKernelGGL(k_xr_update<R>, g(), b(), 0, st, x, r, p, av, n, part, dst);
           scalar(S_END_ITER);
           break;

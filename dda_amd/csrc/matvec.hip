@@ -559,9 +559,42 @@ __global__ __launch_bounds__(BS) void k_pass1s(const typename CxT<R>::T* __restr
   }
 }
 
+// The sums a pass 5 work-group forms over the entries it writes, in double,
+// with x and y as stored: x.y (DOT_U: q0, q1), |y|^2 (DOT_NORM: q0) or |y|^2
+// and x.conj(y) (DOT_OM: q0, q1, q2). Block sums of them, in a fixed order,
+// are stored as a solver's partial sums:
+// quantity q of work-group b of the launch's right-hand side r at
+// dot[(r * 4 + q) * gridDim.x + b]. Every thread of the block calls it.
+template <int BS>
+__device__ inline void dot_partials(double* dot, int op, double q0, double q1, double q2) {
+  __shared__ double red[3][BS / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    q0 += __shfl_xor(q0, off, 64);
+    q1 += __shfl_xor(q1, off, 64);
+    q2 += __shfl_xor(q2, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = q0;
+    red[1][w] = q1;
+    red[2][w] = q2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nq = op == DOT_U ? 2 : op == DOT_NORM ? 1 : 3;
+    double* p = dot + (long long)blockIdx.y * 4 * gridDim.x + blockIdx.x;
+    for (int q = 0; q < nq; q++) {
+      double t = 0;
+      for (int i = 0; i < BS / 64; i++) t += red[q][i];
+      p[(long long)q * gridDim.x] = t;
+    }
+  }
+}
+
 // Pass 5 with a static plan: lines g = c * n_list + slot over the non-empty
 // rows; inverse transform from A1 (loads coalesced), the kept half (x < N/2)
 // through the tile, then line l's dipoles gathered by its TPC threads.
+// dot_op != DOT_NONE: also the partial sums of x and y (dot_partials) into dot.
 template <typename R, int N, bool W, int BS = 256>
 __global__ __launch_bounds__(BS) void k_pass5s(const typename CxT<R>::T* __restrict__ a1,
                                                 const typename CxT<R>::T* __restrict__ x,
@@ -571,7 +604,8 @@ __global__ __launch_bounds__(BS) void k_pass5s(const typename CxT<R>::T* __restr
                                                 const uint32_t* __restrict__ row_dip,
                                                 const uint32_t* __restrict__ row_meta,
                                                 const typename CxT<R>::T* __restrict__ ccs, MvDims d,
-                                                const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl) {
+                                                const typename CxT<R>::T* __restrict__ tw, int conj, RhsFlags fl,
+                                                double* __restrict__ dot, int dot_op) {
   typedef typename CxT<R>::T C;
   typedef StaticPlan<N, W> P;
   constexpr int PPT = P::PPT;
@@ -618,6 +652,7 @@ __global__ __launch_bounds__(BS) void k_pass5s(const typename CxT<R>::T* __restr
   for (int e = 0; e < PPT; e++)
     if (!upper_half<Out>(e)) tile[first_point<N, PPT, Out>(t, e) * lp + l] = v[e];
   __syncthreads();
+  double q0 = 0, q1 = 0, q2 = 0;
 #pragma unroll
   for (int k = 0; k < MAXD; k++) {
     if (s0 + t + k * TPC < s1) {
@@ -626,8 +661,16 @@ __global__ __launch_bounds__(BS) void k_pass5s(const typename CxT<R>::T* __restr
       C r = cadd(cmul(ccs[3 * (gm[k] >> 16) + c], tile[(gm[k] & 0xffff) * lp + l]), a);
       if (conj) r = cconj(r);
       y[(size_t)c * d.S + gi[k]] = r;
+      if (dot_op != DOT_NONE) {
+        const double xr = ga[k].x, xi = ga[k].y, yr = r.x, yi = r.y;
+        const bool du = dot_op == DOT_U;
+        q0 += du ? xr * yr - xi * yi : yr * yr + yi * yi;
+        q1 += du ? xr * yi + xi * yr : xr * yr + xi * yi;
+        q2 += xi * yr - xr * yi;
+      }
     }
   }
+  if (dot_op != DOT_NONE) dot_partials<BS>(dot, dot_op, q0, q1, q2);
   if (blockIdx.x == 0) {
     const int pad = d.S - d.nv;
     for (int q = threadIdx.x; q < 3 * pad; q += blockDim.x) {
@@ -1073,7 +1116,7 @@ dda_status launch_pass3_static(dda_op_s* op, const MvView& v, hipStream_t st, Rh
 
 template <typename R, int N, bool W, int BS = 256>
 dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
-                             hipStream_t st, RhsFlags done, int nrhs) {
+                             hipStream_t st, RhsFlags done, int nrhs, const PassDotReq& dr) {
   typedef typename CxT<R>::T C;
   constexpr int TPC = N / StaticPlan<N, W>::PPT;
   constexpr int LN = BS / TPC;
@@ -1090,9 +1133,12 @@ dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const 
     auto k = k_pass5s<R, N, W, BS>;
     DDA_TRY(set_lds(k, lds));
     const int nl = 3 * op->n_nonempty;
-    hipLaunchKernelGGL(k, dim3(nl > 0 ? (nl + LN - 1) / LN : 1, nrhs), dim3(LN * TPC), lds, st, (const C*)v.a1,
-                       (const C*)x, (C*)y, op->row_list, op->n_nonempty, op->row_start, op->row_dip, op->row_meta,
-                       (const C*)v.ccs, d, (const C*)op->tw[0], conj, done);
+    const int nbx = nl > 0 ? (nl + LN - 1) / LN : 1;
+    op->p5_blocks = nbx;
+    double* dot = dr.op != DOT_NONE ? dr.part + (size_t)dr.r0 * 4 * nbx : nullptr;
+    hipLaunchKernelGGL(k, dim3(nbx, nrhs), dim3(LN * TPC), lds, st, (const C*)v.a1, (const C*)x, (C*)y, op->row_list,
+                       op->n_nonempty, op->row_start, op->row_dip, op->row_meta, (const C*)v.ccs, d,
+                       (const C*)op->tw[0], conj, done, dot, dr.op);
   }
   DDA_TRY_HIP(hipGetLastError());
   return DDA_OK;
@@ -1100,18 +1146,19 @@ dda_status launch_x_static_w(dda_op_s* op, const MvView& v, bool inverse, const 
 
 template <typename R, int N>
 dda_status launch_x_static(dda_op_s* op, const MvView& v, bool inverse, const void* x, void* y, int conj,
-                           hipStream_t st, RhsFlags done, int nrhs, const dda_op_s::Tune& tn) {
+                           hipStream_t st, RhsFlags done, int nrhs, const dda_op_s::Tune& tn,
+                           const PassDotReq& dr = PassDotReq()) {
   // default narrow for both precisions: half the LDS per wave, twice the waves
   // per CU (measured 52 -> 39 us at 256^3 FP32); FP32 also 2 and 3, narrow
   // and wide over 512-thread work-groups (twice the lines each)
   const int var = tn.x_wide[inverse];
   if constexpr (sizeof(R) == 4) {
     if (var >= 2)
-      return (var & 1) ? launch_x_static_w<R, N, true, 512>(op, v, inverse, x, y, conj, st, done, nrhs)
-                       : launch_x_static_w<R, N, false, 512>(op, v, inverse, x, y, conj, st, done, nrhs);
+      return (var & 1) ? launch_x_static_w<R, N, true, 512>(op, v, inverse, x, y, conj, st, done, nrhs, dr)
+                       : launch_x_static_w<R, N, false, 512>(op, v, inverse, x, y, conj, st, done, nrhs, dr);
   }
-  return var >= 0 && (var & 1) ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs)
-                               : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs);
+  return var >= 0 && (var & 1) ? launch_x_static_w<R, N, true>(op, v, inverse, x, y, conj, st, done, nrhs, dr)
+                               : launch_x_static_w<R, N, false>(op, v, inverse, x, y, conj, st, done, nrhs, dr);
 }
 
 // Lengths with a static plan; others use the run-time planned kernels.
@@ -1194,7 +1241,8 @@ struct PassTimer {
 // inverse) of the matvec for right-hand sides r0 .. r0 + nrhs - 1 on `st`.
 template <typename R>
 static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, RhsFlags done,
-                                int r0, int nrhs, int p0, int p1, const dda_op_s::Tune& tn) {
+                                int r0, int nrhs, int p0, int p1, const dda_op_s::Tune& tn,
+                                const PassDotReq& dot = PassDotReq()) {
   typedef typename CxT<R>::T C;
   const MvView v = view_of(op, r0);
   const MvDims& d = v.d;
@@ -1328,11 +1376,17 @@ static dda_status launch_passes(dda_op_s* op, const void* xv, void* yv, int conj
   // pass 5
   if (p0 <= 4 && 4 <= p1) {
     PassTimer pt(op, st, 4);
+    PassDotReq dr = dot;
+    dr.r0 += r0;
+    if (dot.op != DOT_NONE && !op->static_x) {
+      set_error("matvec: partial sums in pass 5 need a compile-time planned X pass");
+      return DDA_NOT_SUPPORTED;
+    }
     if (op->static_x) {
       switch (d.Gx) {
 #define X(N)                                                                    \
   case N:                                                                       \
-    DDA_TRY((launch_x_static<R, N>(op, v, true, x, y, conj, st, done, nrhs, tn))); \
+    DDA_TRY((launch_x_static<R, N>(op, v, true, x, y, conj, st, done, nrhs, tn, dr))); \
     break;
         DDA_STATIC_SIZES(X)
 #undef X
@@ -1379,7 +1433,7 @@ static dda_status split_resources(dda_op_s* op) {
 // at 256^3, 707 us split against 642 us on one stream.
 template <typename R>
 dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipStream_t st, const int* done_ptr,
-                         int nrhs, int done_stride) {
+                         int nrhs, int done_stride, double* dot_part, int dot_op) {
   if (nrhs < 1 || nrhs > op->cap_rhs) {
     set_error("matvec: number of right-hand sides exceeds the operator's capacity");
     return DDA_INVALID_ARG;
@@ -1395,22 +1449,27 @@ dda_status launch_matvec(dda_op_s* op, const void* xv, void* yv, int conj, hipSt
   // blocks of kBlockTuneRhs or more run with their own tuning, when there is one
   const dda_op_s::Tune& tn = nrhs >= kBlockTuneRhs && op->tuned_blk ? op->tune_blk : op->tune;
   const bool split = (tn.split < 0 ? op->split_rhs : tn.split != 0) && nrhs >= 2 && !op->prof;
-  if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4, tn);
+  PassDotReq dot;
+  dot.part = dot_part;
+  dot.op = dot_part ? dot_op : DOT_NONE;
+  if (!split) return launch_passes<R>(op, xv, yv, conj, st, done, 0, nrhs, 0, 4, tn, dot);
   DDA_TRY(split_resources(op));
   const int h = (nrhs + 1) / 2;
   hipStream_t aux = op->split_stream;
   DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 0, 1, tn)));
   DDA_TRY_HIP(hipEventRecord(op->split_ev[0], st));
   DDA_TRY_HIP(hipStreamWaitEvent(aux, op->split_ev[0], 0));
-  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 2, 4, tn)));
-  DDA_TRY((launch_passes<R>(op, xv, yv, conj, aux, done, h, nrhs - h, 0, 4, tn)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, st, done, 0, h, 2, 4, tn, dot)));
+  DDA_TRY((launch_passes<R>(op, xv, yv, conj, aux, done, h, nrhs - h, 0, 4, tn, dot)));
   DDA_TRY_HIP(hipEventRecord(op->split_ev[1], aux));
   DDA_TRY_HIP(hipStreamWaitEvent(st, op->split_ev[1], 0));
   return DDA_OK;
 }
 
-template dda_status launch_matvec<float>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int);
-template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int);
+template dda_status launch_matvec<float>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int,
+                                         double*, int);
+template dda_status launch_matvec<double>(dda_op_s*, const void*, void*, int, hipStream_t, const int*, int, int,
+                                          double*, int);
 
 // Generate the octant of the FFT'd, d^3-scaled tensor on the device in FP64:
 // fill one component on the full grid, 3-D FFT (X rows, Y and Z columns), keep

@@ -114,6 +114,53 @@ def test_graph_replay_matches_eager(ctx, kind, mixed, monkeypatch):
     assert np.array_equal(s.get_pol_vec(einc, 1e-9), p_eager)
 
 
+@pytest.mark.parametrize("kind", ["qmr", "bicgstab", "cgnr", "bicg"])
+def test_pass5_sums_match_reduction_kernels(ctx, kind, monkeypatch):
+    """On a grid whose X passes have a compile-time plan (32^3) the sums that
+    follow a matvec (QMR-CS alpha, BiCG-CS mu, CGNR's two norms, BiCGStab's
+    omega) come from the matvec's pass 5; with DDA_NO_FUSED_SUMS=1 from
+    reduction kernels. Both solve like the oracle, and agree with each other
+    for one right-hand side and for a 3-RHS block, on one stream and split
+    over two (the second half's sums at its own right-hand sides)."""
+    print("pass-5 sums", kind, flush=True)
+    g, params, cc, cs, prop, pol = cases.build("sphere_r7", grid=(32, 32, 32))
+    assert tuple(params.grid) == (32, 32, 32)
+    einc = P.einc_plane(params, prop, pol)
+    eps = 1e-10
+    orc = cases.oracle_problem(g, params, cs)
+    st, p_ref, it_ref, _ = orc.solve(kind, einc, eps, max_iter=5000)
+    assert st == 0
+    op = _op(ctx, g, params, cs)
+    s = D.SOLVER_CLASSES[kind](op, max_iter=5000)
+    one = {}
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("DDA_NO_FUSED_SUMS", raising=False)
+        else:
+            monkeypatch.setenv("DDA_NO_FUSED_SUMS", "1")
+        p = s.get_pol_vec(einc, eps)
+        it = s.info["iterations"]
+        assert abs(it - it_ref) <= max(1, it_ref // 10), (fused, it, it_ref)
+        assert cases.rel_err(p, p_ref) <= 1e-8, fused
+        one[fused] = p
+    assert cases.rel_err(one[True], one[False]) <= 1e-8
+    scales = (1.0, 2.0, 0.5j)
+    op.set_couple_constants_batch(np.stack([cs, cs, cs]))
+    for split in (0, 1):
+        v = list(op.variants())
+        v[5] = split
+        op.variants(v)
+        for fused in (True, False):
+            if fused:
+                monkeypatch.delenv("DDA_NO_FUSED_SUMS", raising=False)
+            else:
+                monkeypatch.setenv("DDA_NO_FUSED_SUMS", "1")
+            pb, infos = s.get_pol_vecs(np.stack([c * einc for c in scales]), eps)
+            for r, c in enumerate(scales):
+                assert infos[r]["status"] == D.OK, (split, fused, r)
+                assert cases.rel_err(pb[r], c * one[True]) <= 1e-8, (split, fused, r)
+
+
 def test_breakdown_status_matches_oracle(ctx):
     """A right-hand side with b^T b = 0 started from x0 = 0: BiCG-CS stops with
     BREAKDOWN after 0 iterations (BicgCs.cpp:68), on the device as in the
