@@ -118,11 +118,55 @@ __device__ inline void write_partials(double* part, int nq, double q0, double q1
   }
 }
 
-// Second stage, run by a whole block for one right-hand side: fixed order.
-__device__ inline double sum_partials(const double* part_rhs, int nb, int q) {
-  double s = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part_rhs[q * nb + i];
-  return block_sum(s);
+// Second stage, run by a k_scalar block (kScalarThreads) for one right-hand
+// side: quantities 0 .. nq-1 of its partials, each in a fixed order (thread t
+// adds entries t + j B, B = kScalarThreads, into chain j mod 4; then the
+// chains, the lanes of a wave and the waves in a fixed order), with up to
+// 4 nq loads in flight per thread.
+constexpr int kScalarThreads = 1024;
+__device__ inline double3 sum_partials(const double* __restrict__ part_rhs, int nb, int nq) {
+  constexpr int B = kScalarThreads;
+  double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0};
+  const double* p0 = part_rhs;
+  const double* p1 = part_rhs + nb;
+  const double* p2 = part_rhs + 2 * (long long)nb;
+  int i = threadIdx.x;
+  for (; i + 3 * B < nb; i += 4 * B) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a0[u] += p0[i + u * B];
+      if (nq > 1) a1[u] += p1[i + u * B];
+      if (nq > 2) a2[u] += p2[i + u * B];
+    }
+  }
+  for (; i < nb; i += B) {
+    a0[0] += p0[i];
+    if (nq > 1) a1[0] += p1[i];
+    if (nq > 2) a2[0] += p2[i];
+  }
+  double s0 = (a0[0] + a0[1]) + (a0[2] + a0[3]);
+  double s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+  double s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  __shared__ double red[3][B / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = s0;
+    red[1][w] = s1;
+    red[2][w] = s2;
+  }
+  __syncthreads();
+  double3 r = make_double3(0.0, 0.0, 0.0);
+  for (int k = 0; k < B / 64; k++) {
+    r.x += red[0][k];
+    r.y += red[1][k];
+    r.z += red[2][k];
+  }
+  return r;
 }
 
 template <typename C> __device__ inline C coef(const double2* p) { return cmake<C>(p->x, p->y); }
@@ -417,7 +461,7 @@ __device__ inline void loop_head(SolverState* st, int kind) {
   st->skip = st->done;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, const double* part_all, int nb, int step,
+__global__ __launch_bounds__(kScalarThreads) void k_scalar(SolverState* st_all, const double* part_all, int nb, int step,
                                                         int kind, double* hist_all) {
   const int rhs = blockIdx.x;
   SolverState* st = st_all + rhs;
@@ -425,7 +469,7 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
   double* hist = hist_all ? hist_all + (long long)rhs * st->hist_cap : nullptr;
   if (st->done) return;  // finished, or masked out of this solve
   // All threads take part in the reductions; thread 0 does the algebra.
-  double q0 = 0, q1 = 0, q2 = 0;
+  int nq = 0;
   switch (step) {
     case S_INIT_TEMP:
     case S_INIT_R:
@@ -434,7 +478,7 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
     case S_CG_RO:
     case S_CG_ALPHA:
     case S_BS_S:
-      q0 = sum_partials(part, nb, 0);
+      nq = 1;
       break;
     case S_QMR_INIT:
     case S_QMR_ALPHA:
@@ -442,18 +486,22 @@ __global__ __launch_bounds__(kRedThreads) void k_scalar(SolverState* st_all, con
     case S_BS_ALPHA:
     case S_BI_RO:
     case S_BI_MU:
-      q0 = sum_partials(part, nb, 0);
-      q1 = sum_partials(part, nb, 1);
+      nq = 2;
       break;
     case S_QMR_GIVENS:
     case S_BS_OMEGA:
     case S_BS_RO1:
-      q0 = sum_partials(part, nb, 0);
-      q1 = sum_partials(part, nb, 1);
-      q2 = sum_partials(part, nb, 2);
+      nq = 3;
       break;
     default:
       break;
+  }
+  double q0 = 0, q1 = 0, q2 = 0;
+  if (nq > 0) {
+    const double3 t = sum_partials(part, nb, nq);
+    q0 = t.x;
+    q1 = t.y;
+    q2 = t.z;
   }
   if (threadIdx.x != 0) return;
   switch (step) {
@@ -886,7 +934,7 @@ struct Core {
     return launch_matvec<R>(op, x, y, conj, st, flag(ioff), nrhs, kStateInts);
   }
   void scalar(int step) {
-    hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kRedThreads), 0, st, dst, part, nb, step, kind, hist);
+    hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kScalarThreads), 0, st, dst, part, nb, step, kind, hist);
   }
   // y = A x, then scalar step `step` over the partial sums `dop` of x and y:
   // formed by the matvec's pass 5 when fused, else by a reduction kernel
@@ -894,7 +942,7 @@ struct Core {
   dda_status matvec_sums(const C* x, C* y, int conj, int ioff, int dop, int step) {
     if (fused) {
       DDA_TRY(launch_matvec<R>(op, x, y, conj, st, flag(ioff), nrhs, kStateInts, part5, dop));
-      hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kRedThreads), 0, st, dst, part5, op->p5_blocks, step, kind,
+      hipLaunchKernelGGL(k_scalar, dim3(nrhs), dim3(kScalarThreads), 0, st, dst, part5, op->p5_blocks, step, kind,
                          hist);
       return DDA_OK;
     }
