@@ -54,6 +54,31 @@ def test_mixed_c1_to_1e8(ctx):
     assert abs(P.abs_cross(params, p, cc) / P.abs_cross(params, p64, cc) - 1) < 1e-7
 
 
+@pytest.mark.parametrize("kind", ["bicgstab", "qmr"])
+def test_mixed_solve_on_1024_point_columns(ctx, kind):
+    """The largest-grid configuration's kernels at a small size: a box of
+    8 x 8 x 500 dipoles (m = 1.6+0.05i, |m| kd = 0.5) on a 32 x 32 x 1024
+    grid, so every Z pass runs on 1024-point columns (FP32: the 32 x 32 plan).
+    Mixed solves to 1e-6 reach their true residual, and their Cext and Cabs
+    agree with an FP64 QMR-CS solve to 1e-10 within 1e-5."""
+    print("mixed 1024-point columns", kind, flush=True)
+    m, d = 1.6 + 0.05j, 1e-7
+    kd = 0.5 / abs(m)
+    g = P.box(8, 8, 500, m, d)
+    params = P.DDAParams(g, 2 * np.pi * d / kd, (32, 32, 1024))
+    assert tuple(params.grid) == (32, 32, 1024)
+    prop, pol1, _ = P.beam_pols((0, 0, 1))
+    cc, cs = P.couple_constants(params, prop, pol1)
+    einc = P.einc_plane(params, prop, pol1)
+    op = _op(ctx, g, params, cs)
+    p64 = D.GpuQmrCs(op, max_iter=20000).get_pol_vec(einc, 1e-10)
+    s = D.SOLVER_CLASSES[kind](op, max_iter=20000, mixed=True)
+    p = s.get_pol_vec(einc, 1e-6)
+    assert s.info["true_rel_residual"] < 1e-6 and s.info["refinements"] >= 1
+    assert abs(P.ext_cross(params, p, einc) / P.ext_cross(params, p64, einc) - 1) < 1e-5
+    assert abs(P.abs_cross(params, p, cc) / P.abs_cross(params, p64, cc) - 1) < 1e-5
+
+
 def test_mixed_c2_block_cext(ctx):
     """Sphere 128 dipoles across, 256^3 grid: both polarisations as one 2-RHS
     block, mixed BiCGStab to 1e-6; Cext and Cabs within 1e-5 of FP64 QMR-CS
