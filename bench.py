@@ -365,17 +365,18 @@ def serving_bench(args, D):
     # with the submitting one)
     with ThreadPoolExecutor(4) as ex:
         f_burst = ex.submit(S.make_stream, args.serve_burst, 42)
-        f_streams = [ex.submit(S.make_stream, args.serve_jobs, 43 + i) for i in range(len(loads))]
+        f_streams = [ex.submit(S.make_stream, args.serve_jobs, 43 + i, None, "stream %d" % (43 + i))
+                     for i in range(len(loads))]
         burst_jobs = f_burst.result()
         streams = [f.result() for f in f_streams]
     srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
-        burst = S.run_stream(srv, burst_jobs)
+        burst = S.run_stream(srv, burst_jobs, keep_pol=False, label="burst")
         cap = burst["jobs"] / burst["makespan_s"]
         out["capacity"] = dict(burst, capacity_jobs_per_s=cap)
         for i, load in enumerate(loads):
-            r = S.run_stream(srv, streams[i], rate=load * cap)
+            r = S.run_stream(srv, streams[i], rate=load * cap, keep_pol=False, label="%.1fx" % load)
             out["poisson_%.1fx" % load] = dict(r, rate_jobs_per_s=load * cap)
             print("serving %.1fx: %s" % (load, json.dumps(r)), file=sys.stderr, flush=True)
             streams[i] = None
@@ -386,7 +387,7 @@ def serving_bench(args, D):
     srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps, pack_grids=True)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))
-        out["capacity_pack_grids"] = S.run_stream(srv, burst_jobs)
+        out["capacity_pack_grids"] = S.run_stream(srv, burst_jobs, keep_pol=False, label="burst, packed")
     finally:
         srv.close()
     ref = out["poisson_0.9x"]

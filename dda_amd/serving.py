@@ -10,6 +10,7 @@ frame, LDR polarisabilities. Everything is drawn from numpy's PCG64 with the
 given seed.
 """
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -69,10 +70,15 @@ def make_job(rng, shape=None, dim=None, m=None, other=None):
                np.array([s for _, s in ccs]))
 
 
-def make_stream(n, seed, rate=None):
-    """n jobs; with `rate` (jobs/s) Poisson arrival times, else all at t = 0."""
+def make_stream(n, seed, rate=None, label=None):
+    """n jobs; with `rate` (jobs/s) Poisson arrival times, else all at t = 0.
+    `label`: a progress line on stderr every 500 jobs made."""
     rng = np.random.default_rng(seed)
-    jobs = [make_job(rng) for _ in range(n)]
+    jobs = []
+    for k in range(n):
+        jobs.append(make_job(rng))
+        if label and (k + 1) % 500 == 0:
+            print("serving %s: %d of %d jobs made" % (label, k + 1, n), file=sys.stderr, flush=True)
     t = 0.0
     for j in jobs:
         if rate:
@@ -89,10 +95,13 @@ def submit(server, job, grid=(0, 0, 0)):
                          nmat=len(g.materials), grid=grid)
 
 
-def run_stream(server, jobs, rate=None):
+def run_stream(server, jobs, rate=None, keep_pol=True, label=None):
     """Submits the jobs at their arrival times (re-drawn for `rate` if given,
     keeping the order), waits for all and returns latency and throughput
-    figures: latency = completion - submission on the server's clock."""
+    figures: latency = completion - submission on the server's clock.
+    keep_pol=False drops each job's polarisations once it is done (long
+    streams); `label`: a progress line on stderr every 250 jobs submitted and
+    every 250 jobs done."""
     if rate:
         rng = np.random.default_rng(12345)
         t = 0.0
@@ -108,9 +117,13 @@ def run_stream(server, jobs, rate=None):
                 break
             time.sleep(min(dt, 0.002))
         ids.append(submit(server, j))
-    for j, i in zip(jobs, ids):
+        if label and len(ids) % 250 == 0:
+            print("serving %s: %d of %d jobs submitted" % (label, len(ids), len(jobs)), file=sys.stderr, flush=True)
+    for k, (j, i) in enumerate(zip(jobs, ids)):
         pol, infos, res = server.wait(i)
-        j.result = {"pol": pol, "infos": infos, **res}
+        j.result = {"pol": pol if keep_pol else None, "infos": infos, **res}
+        if label and (k + 1) % 250 == 0:
+            print("serving %s: %d of %d jobs done" % (label, k + 1, len(jobs)), file=sys.stderr, flush=True)
     lat = np.array([j.result["t_end"] - j.result["t_submit"] for j in jobs])
     start = min(j.result["t_submit"] for j in jobs)
     end = max(j.result["t_end"] for j in jobs)
