@@ -356,7 +356,7 @@ def serving_bench(args, D):
     of use. Runs in its own process (--serve-only), as a server would."""
     from concurrent.futures import ThreadPoolExecutor
     from dda_amd import serving as S
-    loads = (0.5, 0.9, 1.2)
+    loads = tuple(float(x) for x in args.serve_loads.split(","))
     out = {"workload": "seeded streams of %d jobs per load (burst of %d for capacity), sphere/box/ellipsoid, extent "
                        "U{16..96}, Re m U(1.2,2), Im m U(0,0.2), |m|kd=0.5, random orientation, 2 polarisations, "
                        "mixed BiCGStab to %g, %d slots" % (args.serve_jobs, args.serve_burst, args.eps,
@@ -390,7 +390,7 @@ def serving_bench(args, D):
         out["capacity_pack_grids"] = S.run_stream(srv, burst_jobs, keep_pol=False, label="burst, packed")
     finally:
         srv.close()
-    ref = out["poisson_0.9x"]
+    ref = out.get("poisson_0.9x") or out["poisson_%.1fx" % loads[-1]]
     out.update({"p50_ms": ref["p50_ms"], "p99_ms": ref["p99_ms"], "solves_per_s": ref["solves_per_s"]})
     return out
 
@@ -401,7 +401,7 @@ def serving_child(args):
     import subprocess
     cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", "--eps", repr(args.eps),
            "--serve-jobs", str(args.serve_jobs), "--serve-burst", str(args.serve_burst),
-           "--serve-slots", str(args.serve_slots)]
+           "--serve-slots", str(args.serve_slots), "--serve-loads", args.serve_loads]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
     if r.returncode != 0 or not lines:
@@ -439,6 +439,8 @@ def main():
                     help="jobs of each seeded Poisson serving stream (0 = skip the serving measurement)")
     ap.add_argument("--serve-burst", type=int, default=48, help="jobs of the capacity burst")
     ap.add_argument("--serve-slots", type=int, default=4, help="worker slots of the serving measurement")
+    ap.add_argument("--serve-loads", default="0.5,0.9,1.2",
+                    help="Poisson arrival rates of the serving streams, as fractions of the burst capacity")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving measurement (one JSON line); the full run starts itself this way")
     ap.add_argument("--no-large-grid", action="store_true",

@@ -125,6 +125,9 @@ def run_stream(server, jobs, rate=None, keep_pol=True, label=None):
         if label and (k + 1) % 250 == 0:
             print("serving %s: %d of %d jobs done" % (label, k + 1, len(jobs)), file=sys.stderr, flush=True)
     lat = np.array([j.result["t_end"] - j.result["t_submit"] for j in jobs])
+    wait = np.array([j.result["t_start"] - j.result["t_submit"] for j in jobs])
+    serve = np.array([j.result["t_end"] - j.result["t_start"] for j in jobs])
+    slow = jobs[int(np.argmax(serve))]
     start = min(j.result["t_submit"] for j in jobs)
     end = max(j.result["t_end"] for j in jobs)
     solves = sum(len(j.result["infos"]) for j in jobs)
@@ -139,4 +142,11 @@ def run_stream(server, jobs, rate=None, keep_pol=True, label=None):
             "setup_ms_mean": 1e3 * float(np.mean([j.result["setup_s"] for j in jobs])),
             "tensor_ms_mean": 1e3 * float(np.mean([j.result["tensor_s"] for j in jobs])),
             "tensor_ms_p50": 1e3 * float(np.percentile([j.result["tensor_s"] for j in jobs], 50)),
-            "tensor_ms_max": 1e3 * float(np.max([j.result["tensor_s"] for j in jobs]))}
+            "tensor_ms_max": 1e3 * float(np.max([j.result["tensor_s"] for j in jobs])),
+            # queueing (submission to a slot taking the job) and service (slot to completion)
+            "wait_ms_p50": 1e3 * float(np.percentile(wait, 50)), "wait_ms_p99": 1e3 * float(np.percentile(wait, 99)),
+            "service_ms_p50": 1e3 * float(np.percentile(serve, 50)),
+            "service_ms_p99": 1e3 * float(np.percentile(serve, 99)), "service_ms_max": 1e3 * float(serve.max()),
+            "slowest_job": {"box": [int(b) for b in slow.geometry.box], "nv": int(slow.geometry.nv),
+                            "m": [float(np.real(slow.m)), float(np.imag(slow.m))],
+                            "iterations": int(slow.result["iterations"])}}
