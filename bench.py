@@ -359,8 +359,8 @@ def serving_bench(args, D):
     loads = tuple(float(x) for x in args.serve_loads.split(","))
     out = {"workload": "seeded streams of %d jobs per load (burst of %d for capacity), sphere/box/ellipsoid, extent "
                        "U{16..96}, Re m U(1.2,2), Im m U(0,0.2), |m|kd=0.5, random orientation, 2 polarisations, "
-                       "mixed BiCGStab to %g, %d slots" % (args.serve_jobs, args.serve_burst, args.eps,
-                                                            args.serve_slots)}
+                       "mixed BiCGStab to %g, %d slots, long solves' GPU share %s" % (
+                           args.serve_jobs, args.serve_burst, args.eps, args.serve_slots, args.serve_share or "off")}
     # every stream is generated before any runs (no host threads competing
     # with the submitting one)
     with ThreadPoolExecutor(4) as ex:
@@ -369,7 +369,8 @@ def serving_bench(args, D):
                      for i in range(len(loads))]
         burst_jobs = f_burst.result()
         streams = [f.result() for f in f_streams]
-    srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps)
+    share = tuple(float(x) for x in args.serve_share.split(",")) if args.serve_share else None
+    srv = D.Server(slots=args.serve_slots, solver="bicgstab", mixed=True, eps=args.eps, long_solve=share)
     try:
         S.run_stream(srv, S.make_stream(2, seed=7))  # warm-up: library and kernel load
         burst = S.run_stream(srv, burst_jobs, keep_pol=False, label="burst")
@@ -401,7 +402,8 @@ def serving_child(args):
     import subprocess
     cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", "--eps", repr(args.eps),
            "--serve-jobs", str(args.serve_jobs), "--serve-burst", str(args.serve_burst),
-           "--serve-slots", str(args.serve_slots), "--serve-loads", args.serve_loads]
+           "--serve-slots", str(args.serve_slots), "--serve-loads", args.serve_loads, "--serve-share",
+           args.serve_share]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
     if r.returncode != 0 or not lines:
@@ -439,6 +441,9 @@ def main():
                     help="jobs of each seeded Poisson serving stream (0 = skip the serving measurement)")
     ap.add_argument("--serve-burst", type=int, default=48, help="jobs of the capacity burst")
     ap.add_argument("--serve-slots", type=int, default=4, help="worker slots of the serving measurement")
+    ap.add_argument("--serve-share", default="0.5,0.25",
+                    help="SECONDS,SHARE: the server's solvers yield the GPU past SECONDS (dda_solver_set_gpu_share; "
+                         "empty = off)")
     ap.add_argument("--serve-loads", default="0.5,0.9,1.2",
                     help="Poisson arrival rates of the serving streams, as fractions of the burst capacity")
     ap.add_argument("--serve-only", action="store_true",

@@ -40,7 +40,7 @@ EXPORTS = [
     "dda_op_apply_host", "dda_op_get_dmatrix", "dda_op_profile", "dda_op_profile_read",
     "dda_solver_create", "dda_solver_create_ex", "dda_solver_destroy", "dda_solver_solve", "dda_solver_solve_batch",
     "dda_solver_history",
-    "dda_solver_set_inner_floor", "dda_solver_set_inner_plateau", "dda_solver_profile_read",
+    "dda_solver_set_inner_floor", "dda_solver_set_inner_plateau", "dda_solver_set_gpu_share", "dda_solver_profile_read",
     "dda_solver_profiling_run",
     "dda_dhat_cache_create", "dda_dhat_cache_destroy", "dda_dhat_cache_stats", "dda_op_create_cached",
     "dda_server_create", "dda_server_submit", "dda_server_wait", "dda_server_stats", "dda_server_destroy",
@@ -85,7 +85,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("cache_bytes", ctypes.c_size_t), ("solver_kind", ctypes.c_int), ("mode", ctypes.c_int),
                 ("eps", ctypes.c_double), ("max_iter", ctypes.c_uint64), ("max_pack", ctypes.c_int),
-                ("pack_grids", ctypes.c_int)]
+                ("pack_grids", ctypes.c_int), ("long_solve_s", ctypes.c_double),
+                ("long_solve_share", ctypes.c_double)]
 
 
 class JobDesc(ctypes.Structure):
@@ -149,6 +150,7 @@ def lib():
         L.dda_solver_create_ex.argtypes = [vp, i32, u64, u64, i32, pp]
         L.dda_solver_set_inner_floor.argtypes = [vp, dbl]
         L.dda_solver_set_inner_plateau.argtypes = [vp, u64, dbl]
+        L.dda_solver_set_gpu_share.argtypes = [vp, dbl, dbl]
         L.dda_solver_profile_read.argtypes = [vp, dp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.dda_solver_destroy.argtypes = [vp]
         L.dda_solver_profiling_run.argtypes = [vp, dp, dp, dp]
@@ -568,6 +570,12 @@ class IterativeSolver:
         self.info = None
         gpu_matvec.ctx._adopt(self)
 
+    def set_gpu_share(self, after_seconds, share):
+        """dda_solver_set_gpu_share: past after_seconds of a solve, leave the
+        GPU to other streams for (1 / share - 1) of the time (share 1 = off)."""
+        _check(lib().dda_solver_set_gpu_share(self.handle, float(after_seconds), float(share)),
+               "dda_solver_set_gpu_share")
+
     def set_couple_constants(self, cc_sqrt):
         self.op.set_couple_constants(cc_sqrt)
 
@@ -703,12 +711,16 @@ class TensorCache:
 class Server:
     """Scattering-job server (dda_server_*): queue, HBM admission, `slots`
     concurrent block solves on their own streams, tensor cache, packing of
-    jobs of the same particle (with pack_grids, of any particles on one grid)."""
+    jobs of the same particle (with pack_grids, of any particles on one grid).
+    long_solve = (seconds, share): every job's solver yields the GPU to the
+    other slots once it has run that long (dda_solver_set_gpu_share)."""
 
     def __init__(self, device=0, slots=4, solver="bicgstab", mixed=True, eps=1e-6, max_iter=0, max_pack=16,
-                 hbm_budget=0, cache_bytes=0, pack_grids=False):
+                 hbm_budget=0, cache_bytes=0, pack_grids=False, long_solve=None):
         c = ServerConfig()
         c.pack_grids = int(bool(pack_grids))
+        if long_solve is not None:
+            c.long_solve_s, c.long_solve_share = float(long_solve[0]), float(long_solve[1])
         c.device, c.slots, c.hbm_budget, c.cache_bytes = device, slots, int(hbm_budget), int(cache_bytes)
         c.solver_kind = SOLVER_KINDS[solver]
         c.mode = MODE_MIXED if mixed else MODE_FP64

@@ -23,6 +23,7 @@
 // vectors are [nrhs][n] back to back.
 
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -926,6 +927,12 @@ struct Core {
   // plateau stop for the next solves (SolverState::plateau_window; 0 = off)
   uint64_t plateau_window = 0;
   double plateau_ratio = 0.81;
+  // GPU share of a long solve (dda_solver_set_gpu_share): once a solve has run
+  // share_after_s seconds, each chunk is waited for and the stream then left
+  // idle for (1 / share - 1) times as long, so that other streams' work runs
+  // in between; share 1 = off
+  double share_after_s = 0;
+  double share = 1.0;
 
   dim3 g() const { return dim3(nb, nrhs); }
   dim3 b() const { return dim3(kRedThreads); }
@@ -1249,6 +1256,14 @@ struct Core {
         len = std::max(len, li);
       }
       next_len = len;
+      if (!finished && share < 1.0) {
+        const auto t_now = std::chrono::steady_clock::now();
+        if (std::chrono::duration<double>(t_now - t_start).count() >= share_after_s) {
+          DDA_TRY_HIP(hipStreamSynchronize(st));  // the chunk queued behind the one just read
+          const double busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_now).count();
+          std::this_thread::sleep_for(std::chrono::duration<double>(busy * (1.0 / share - 1.0)));
+        }
+      }
       if (progress_s > 0) {
         const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         if (now - last_print >= progress_s) {
@@ -1795,6 +1810,16 @@ extern "C" dda_status dda_solver_set_inner_plateau(dda_solver_t s, uint64_t wind
   }
   s->inner_plateau = window;
   s->inner_plateau_ratio = ratio;
+  return DDA_OK;
+}
+
+extern "C" dda_status dda_solver_set_gpu_share(dda_solver_t s, double after_seconds, double share) {
+  if (!s || !(after_seconds >= 0) || !(share > 0 && share <= 1)) {
+    set_error("dda_solver_set_gpu_share: after_seconds >= 0 and a share in (0, 1]");
+    return DDA_INVALID_ARG;
+  }
+  for (double* p : {&s->c64.share_after_s, &s->c32.share_after_s}) *p = after_seconds;
+  for (double* p : {&s->c64.share, &s->c32.share}) *p = share;
   return DDA_OK;
 }
 

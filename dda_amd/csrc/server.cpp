@@ -193,6 +193,8 @@ void dda_server_s::run_multi(dda_ctx_t ctx, std::vector<Job*>& pack, const std::
   if (rc == DDA_OK) rc = dda_op_set_cc_sqrt_batch(op, cc.data(), k);
   if (rc == DDA_OK) rc = dda_op_set_rhs_members(op, member.data(), k);
   if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
+  if (rc == DDA_OK && cfg.long_solve_share > 0 && cfg.long_solve_share < 1)
+    rc = dda_solver_set_gpu_share(s, cfg.long_solve_s, cfg.long_solve_share);
   const double setup = now() - t0;
   const double tensor_s = dda::tensor_cache_seconds() - tc0;  // FP64 tensor and (mixed) its FP32 twin
   if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
@@ -268,6 +270,8 @@ void dda_server_s::run_pack(dda_ctx_t ctx, std::vector<Job*>& pack) {
   const bool hit = rc == DDA_OK && dda::last_tensor_cache_hit();
   if (rc == DDA_OK) rc = dda_op_set_cc_sqrt_batch(op, cc.data(), k);
   if (rc == DDA_OK) rc = dda_solver_create_ex(op, cfg.solver_kind, cfg.max_iter, 0, cfg.mode, &s);
+  if (rc == DDA_OK && cfg.long_solve_share > 0 && cfg.long_solve_share < 1)
+    rc = dda_solver_set_gpu_share(s, cfg.long_solve_s, cfg.long_solve_share);
   const double setup = now() - t0;
   const double tensor_s = dda::tensor_cache_seconds() - tc0;  // FP64 tensor and (mixed) its FP32 twin
   if (rc == DDA_OK) rc = dda_solver_solve_batch(s, einc.data(), k, cfg.eps, nullptr, pol.data(), info.data());
@@ -377,7 +381,8 @@ void dda_server_s::worker(int slot) {
 
 extern "C" dda_status dda_server_create(const dda_server_config* cfg, dda_server_t* out) {
   if (!cfg || !out || cfg->slots < 1 || cfg->slots > 16 || cfg->max_pack < 1 || !(cfg->eps > 0) ||
-      cfg->solver_kind < 0 || cfg->solver_kind > 3 || (cfg->mode != DDA_MODE_FP64 && cfg->mode != DDA_MODE_MIXED)) {
+      cfg->solver_kind < 0 || cfg->solver_kind > 3 || (cfg->mode != DDA_MODE_FP64 && cfg->mode != DDA_MODE_MIXED) ||
+      !(cfg->long_solve_s >= 0) || !(cfg->long_solve_share >= 0 && cfg->long_solve_share <= 1)) {
     dda::set_error("dda_server_create: need 1..16 slots, max_pack >= 1, eps > 0, a solver kind and mode");
     return DDA_INVALID_ARG;
   }

@@ -277,6 +277,13 @@ dda_status dda_solver_set_inner_floor(dda_solver_t s, double floor_);
  * its value `window` iterations earlier. Default window 200, ratio 0.81 (|r|
  * down by less than 10%); window 0 turns it off. */
 dda_status dda_solver_set_inner_plateau(dda_solver_t s, uint64_t window, double ratio);
+/* GPU share of long solves (no counterpart in the reference; for servers that
+ * run several solves on one GPU): once a solve (in mixed mode: an inner or a
+ * refinement solve) has run after_seconds, the host waits for each chunk of
+ * iterations and leaves the stream idle for (1 / share - 1) times as long
+ * before the next, so other streams' work runs in between. share 1 (the
+ * default) turns it off; results do not depend on it. */
+dda_status dda_solver_set_gpu_share(dda_solver_t s, double after_seconds, double share);
 /* dda_op_profile_read for the operator the Krylov iteration runs on: the FP32
  * twin in mixed mode (it follows the FP64 operator's dda_op_profile switch),
  * else the FP64 operator itself. */
@@ -407,6 +414,8 @@ typedef struct dda_server_config {
   int pack_grids;      /* 1: also pack different particles on one grid and grid unit
                           (dda_op_create_pack; a job's result then equals its solve
                           alone to the solver's tolerance, not bit for bit) */
+  double long_solve_s;     /* with long_solve_share in (0, 1): dda_solver_set_gpu_share */
+  double long_solve_share; /* of every job's solver (0 or 1 = off) */
 } dda_server_config;
 typedef struct dda_job_desc {
   dda_op_desc geometry;  /* as for dda_op_create; dmatrix must be NULL */

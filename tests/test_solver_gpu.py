@@ -161,6 +161,24 @@ def test_pass5_sums_match_reduction_kernels(ctx, kind, monkeypatch):
                 assert cases.rel_err(pb[r], c * one[True]) <= 1e-8, (split, fused, r)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_gpu_share_leaves_results_unchanged(ctx, mixed):
+    """dda_solver_set_gpu_share only spaces the chunks of a long solve out in
+    time: with it on from the first chunk (after 0 s, share 0.5) the solve
+    gives bitwise the result and the iteration count it gives without it."""
+    g, params, cc, cs, prop, pol = cases.build("sphere_r7", grid=(32, 32, 32))
+    einc = P.einc_plane(params, prop, pol)
+    s = D.GpuBicgStab(_op(ctx, g, params, cs), max_iter=5000, mixed=mixed)
+    p0 = s.get_pol_vec(einc, 1e-9)
+    it0 = s.info["iterations"]
+    s.set_gpu_share(0.0, 0.5)
+    p1 = s.get_pol_vec(einc, 1e-9)
+    assert s.info["iterations"] == it0 and np.array_equal(p1, p0)
+    s.set_gpu_share(0.0, 1.0)
+    with pytest.raises(D.DDAError):
+        s.set_gpu_share(0.0, 0.0)
+
+
 def test_breakdown_status_matches_oracle(ctx):
     """A right-hand side with b^T b = 0 started from x0 = 0: BiCG-CS stops with
     BREAKDOWN after 0 iterations (BicgCs.cpp:68), on the device as in the
